@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X photon-transport path.
+
+Workload (the multilayer, many-detector configuration): three media -- RGD
+r = 0.1 um (mu_s 5, mu_a 0.02 /mm, 0.5 mm), Mie r = 1.0 um (mu_s 20, mu_a
+0.005 /mm, 0.5 mm) over a semi-infinite HG g = 0.9 base (mu_s 10, mu_a 0.01
+/mm) -- uniform n = 1.33, circularly polarised Gaussian beam (sigma 0.05 mm),
+sensors at z = 0: far-field CBS 256 theta x 64 phi, planar fluence 256 x 256 x
+(64 + 1) time bins over +-5 mm and 0-200 ps, far-field fluence 180 x 90 and
+total reflectance; 1e8 photons per step, FP32 transport with FP64
+accumulation.  One step = one lmc_scene_run over the step's photon budget,
+with the scene (tables, accumulators) resident in HBM.
+
+Prints one JSON line (see the repository's benchmark contract).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "luminis-mc_amd"), ROOT, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+# Algorithmic work per unit (frozen model): one scattering event with CBS
+# tracking = 266 FLOP (free path + move 14, theta 14, S(theta) 14, phi setup 33,
+# phi trial 12 x 1.4, frame 51, Jones 41, T_raw + renorm 78, weight 4); each
+# extra azimuth trial 12; light sensor deposit (fluence/field/far-field) 40;
+# CBS reverse-path reconstruction 700 (counted in the records kernel).
+F_EVENT = 266.0
+F_TRIAL = 12.0
+F_DEPOSIT = 40.0
+F_CBS = 700.0
+PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector, dual-issue/packed
+PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no packed FP32)
+
+
+def metric_name():
+    return "photons/s/GPU at matched CBS-bin σ: plain vs batched+adjoint; speedup vs CPU"
+
+
+def flops_of(st):
+    extra_trials = max(0, st["phi_trials"] - st["events"])
+    transport = st["events"] * F_EVENT + extra_trials * F_TRIAL + st["crossings"] * F_DEPOSIT
+    return transport, st["cbs_detections"] * F_CBS
+
+
+def cpu_baseline(build, seconds: float):
+    import oracle
+    threads = os.cpu_count() or 1
+    n_cal = 20000
+    t = time.perf_counter()
+    oracle.run(build, n_cal, 12345, n_threads=threads)
+    rate = n_cal / (time.perf_counter() - t)
+    n = int(min(max(rate * seconds, n_cal), 5e7))
+    t = time.perf_counter()
+    oracle.run(build, n, 777, n_threads=threads, photon_offset=10 ** 9)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"{n} photons of the same multilayer workload on the FP64 C++ restatement "
+                      f"(xoshiro per-photon streams, std::thread clone/merge), {dt:.1f} s wall"}
+
+
+def plain_mie_rate(n: int):
+    import scenes
+    from luminis_mc.simulation import GpuScene, run_desc_from
+    cfg, det = scenes.config_plain_mie(n, seed=0)
+    sc = GpuScene(cfg.sample, cfg.laser, cfg.detector.detectors)
+    sc.run(run_desc_from(cfg, n_photons=min(n, 10 ** 6)))
+    t = time.perf_counter()
+    st = sc.run(run_desc_from(cfg))
+    dt = time.perf_counter() - t
+    sc.close()
+    return n / dt, st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the transport path has no multi-GPU mode in this build "
+              "(photon ranges would shard across GPUs with one final FP64 reduction)", file=sys.stderr)
+        sys.exit(2)
+
+    import luminis_mc as lmc
+    import scenes
+    from luminis_mc.simulation import GpuScene, run_desc_from
+
+    n = int(args.photons)
+    cfg, sensors = scenes.config_multilayer(n, seed=2026, small=False)
+    scene = GpuScene(cfg.sample, cfg.laser, cfg.detector.detectors)
+    for w in range(args.warmup):
+        scene.run(run_desc_from(cfg, n_photons=n, photon_offset=(10 ** 12) + w * n))
+    scene.zero()
+    tot = {"photons": 0, "events": 0, "phi_trials": 0, "crossings": 0, "cbs_detections": 0,
+           "transport_ms": 0.0, "records_ms": 0.0, "kernel_ms": 0.0, "transport_launches": 0}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = scene.run(run_desc_from(cfg, n_photons=n, photon_offset=k * n))
+        for key in tot:
+            tot[key] += st[key]
+    elapsed = time.perf_counter() - t0
+    value = args.steps * n / elapsed
+    acc, hits = scene.read()
+    f_transport, f_cbs = flops_of(tot)
+    avg_launch_s = tot["transport_ms"] * 1e-3 / max(1, tot["transport_launches"])
+    per_launch = f_transport / max(1, tot["transport_launches"])
+    achieved = per_launch / avg_launch_s / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "frac_of_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED, 4), "traffic": None,
+            "kernel": "k_transport", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+            "flop_per_launch": per_launch}
+    out = {
+        "metric": metric_name(), "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (photon transport from a seeded Philox stream; no input data)",
+        "config": {"workload": "multilayer 3-media CBS 256x64 + planar fluence 256x256x65 + far-field "
+                               "fluence 180x90 + reflectance, 1e8 photons/step",
+                   "photons_per_step": n, "events_per_photon": round(tot["events"] / max(1, tot["photons"]), 2),
+                   "kernel_ms_per_step": round(tot["kernel_ms"] / args.steps, 3),
+                   "transport_ms_per_step": round(tot["transport_ms"] / args.steps, 3),
+                   "records_ms_per_step": round(tot["records_ms"] / args.steps, 3),
+                   "reflectance": float(acc[scene.build.offsets[3][0]]) / (args.steps * n)},
+        "roofline": roof,
+    }
+    if not args.no_plain:
+        rate, pst = plain_mie_rate(10 ** 7)
+        out["plain_mie_cbs"] = {"photons_per_s": round(rate, 1), "photons": 10 ** 7,
+                                "events_per_photon": round(pst["events"] / max(1, pst["photons"]), 2),
+                                "kernel_ms": round(pst["kernel_ms"], 3)}
+    if not args.no_cpu:
+        cb = cpu_baseline(scene.build, args.cpu_seconds)
+        out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}
+        out["speedup_vs_cpu"] = round(value / cb["value"], 2)
+    scene.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

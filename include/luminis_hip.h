@@ -1,0 +1,286 @@
+/*
+ * luminis_hip.h -- C ABI of the MI355X (gfx950) photon-transport library.
+ *
+ * This is the drop-in boundary for luminis-mc's transport hot path.  The
+ * reference reaches that path through one pybind11 call,
+ *   run_simulation_parallel(config)         python/python_module.cpp:664-673
+ *                                           -> src/simulation.cpp:105-249
+ * which takes a SimConfig holding raw pointers to a Sample, a Laser and a
+ * SensorsGroup (include/luminis/core/simulation.hpp:67-90) and *adds* into
+ * the sensors' accumulators (src/detector.cpp:283-297).  Here the same
+ * objects are flattened by the host into plain-old-data descriptors
+ * (lmc_scene_desc), uploaded once (lmc_scene_create), run any number of times
+ * (lmc_scene_run adds into device-resident FP64 accumulators) and read back
+ * by adding into caller-owned host buffers laid out exactly like the
+ * reference's Matrix/CMatrix grids (include/luminis/math/vec.hpp:116-170).
+ *
+ * Plain C: status codes, no exceptions, no torch types, caller-owned buffers.
+ * Every call is synchronous unless documented otherwise.  A scene object is
+ * not re-entrant (like a SensorsGroup, src/simulation.cpp:105).
+ */
+#ifndef LUMINIS_HIP_H
+#define LUMINIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMC_ABI_VERSION 1
+
+/* Status codes (reference: std::invalid_argument -> ValueError,
+ * src/simulation.cpp:95-103; fatal exits become LMC_ERR_*). */
+enum {
+  LMC_OK = 0,
+  LMC_ERR_INVALID = 1,     /* bad descriptor / argument (ValueError)      */
+  LMC_ERR_DEVICE = 2,      /* HIP runtime failure / no device             */
+  LMC_ERR_NOMEM = 3,       /* device allocation failed                    */
+  LMC_ERR_UNSUPPORTED = 4  /* valid but not implemented on this path      */
+};
+
+/* ---- theta samplers (include/luminis/sample/phase.hpp; src/phase.cpp) ---- */
+enum {
+  LMC_PHASE_TABLE_THETA = 0, /* SamplingTable over theta: Mie, RayleighDebye(EMC)
+                                (src/phase.cpp:132-137,162-167,262-268)        */
+  LMC_PHASE_TABLE_COS = 1,   /* SamplingTable over cos(theta); theta = acos:
+                                Rayleigh, Draine (src/phase.cpp:90-95,230-235) */
+  LMC_PHASE_HG = 2,          /* Henyey-Greenstein closed form (src/phase.cpp:108-115) */
+  LMC_PHASE_UNIFORM = 3      /* cos = 2u-1 (src/phase.cpp:78-83)               */
+};
+
+/* One scattering species: a ScatteringMedium plus its PhaseFunction
+ * (include/luminis/core/medium.hpp:71-189).  All tables live in the
+ * scene-wide `tables` array; offsets are in doubles. */
+typedef struct lmc_species_desc {
+  int32_t phase_kind;   /* LMC_PHASE_*                                        */
+  int32_t cdf_n;        /* SamplingTable nodes (= nDiv+1), table kinds only    */
+  int64_t cdf_offset;   /* normalised CDF, cdf_n doubles (src/table.cpp:6-48)  */
+  double cdf_min;       /* node abscissa i = cdf_min + i*cdf_step              */
+  double cdf_step;
+  double hg_g;          /* LMC_PHASE_HG asymmetry g (must be != 0)             */
+  int32_t s_n;          /* DataTable nodes on theta_i = i*s_dtheta             */
+  int32_t reserved0;
+  int64_t s1_offset;    /* S1(theta): s_n complex values, re/im interleaved    */
+  int64_t s2_offset;    /* S2(theta): idem  (src/medium.cpp:99-121,203-236)   */
+  double s_dtheta;
+  double mu_s, mu_a;    /* species coefficients [1/mm]                         */
+  double mfp;           /* species mean_free_path field [mm] (medium.cpp:123-126) */
+  double sigma_s;       /* scattering cross-section (mixtures only)            */
+} lmc_species_desc;
+
+/* One z-slab of the Sample (include/luminis/core/sample.hpp:68-204).
+ * species_count == 1: HomogeneousLayer; > 1: MixtureLayer whose normalised
+ * selection CDF (species_count doubles) sits at tables[select_offset]. */
+typedef struct lmc_layer_desc {
+  double z_min, z_max;  /* [z_min, z_max); z_max may be +inf                  */
+  double mu_s, mu_a, mu_t; /* aggregate coefficients (albedo split, interface
+                              optical-depth rescale: simulation.cpp:336,373,580-587) */
+  double mfp;           /* free-path mean of the layer (HomogeneousLayer:
+                           medium mfp; MixtureLayer: 1/mu_s_total, sample.cpp:77) */
+  double n;             /* refractive index used for the optical path (uniform
+                           host index in the reference, sample.cpp:110-116) */
+  int32_t species_first;
+  int32_t species_count;
+  int64_t select_offset;
+} lmc_layer_desc;
+
+/* Sensor kinds (include/luminis/core/detector.hpp:280-699). */
+enum {
+  LMC_SENSOR_FARFIELD_CBS = 1,   /* FarFieldCBSSensor, analog process_hit      */
+  LMC_SENSOR_PLANAR_FLUENCE = 2, /* PlanarFluenceSensor                        */
+  LMC_SENSOR_PLANAR_FIELD = 3,   /* PlanarFieldSensor                          */
+  LMC_SENSOR_FARFIELD_FLUENCE = 4, /* extension: exit-direction Stokes histogram
+                                      (no reference counterpart)             */
+  LMC_SENSOR_REFLECTANCE = 5     /* extension: sum of exit weights (the
+                                    PhotonRecordSensor weight sum)          */
+};
+
+/* Crossing-direction filter values (detector.hpp CrossingDirection). */
+enum { LMC_DIR_FORWARD = 0, LMC_DIR_BACKWARD = 1, LMC_DIR_BOTH = 2 };
+
+/* One sensor plane with its filters (Sensor base, detector.hpp:95-180,
+ * check_conditions detector.cpp:374-417).  Grid geometry by kind:
+ *  CBS:       g[0..5] = theta_max, phi_max, t_max, dtheta, dphi, dt;
+ *             n[0..2] = N_theta, N_phi, N_t.  Output: 8 grids
+ *             [S0c,S1c,S2c,S3c,S0i,S1i,S2i,S3i][t][theta][phi] doubles.
+ *  FLUENCE:   g = len_x, len_y, len_t, dx, dy, dt; n = N_x, N_y, N_t.
+ *             Output: 4 grids [S0..S3][t][x][y] doubles.
+ *  FIELD:     g = len_x, len_y, dx, dy; n = N_x, N_y.  Output: Ex grid then
+ *             Ey grid, [x][y] complex (re, im) doubles.
+ *  FF_FLUENCE g = theta_max, phi_max, dtheta, dphi; n = N_theta, N_phi.
+ *             Output: 4 grids [S0..S3][theta][phi].
+ *  REFLECTANCE: output 1 double (sum of weights). */
+typedef struct lmc_sensor_desc {
+  int32_t kind;
+  int32_t absorb;
+  double z;
+  int32_t f_dir_en, f_dir;
+  int32_t f_ev_en, f_ev_min, f_ev_max;
+  int32_t f_th_en;
+  double cos_th_min, cos_th_max; /* cached cosine window (detector.cpp:336-345) */
+  int32_t f_ph_en, reserved0;
+  double ph_min, ph_max;
+  int32_t f_pos_en, reserved1;
+  double x_min, x_max, y_min, y_max;
+  double g[8];
+  int32_t n[4];
+  int64_t out_offset;   /* offset (doubles) of this sensor's accumulators     */
+  int64_t out_len;      /* number of doubles                                  */
+} lmc_sensor_desc;
+
+/* Laser (include/luminis/core/laser.hpp:45-90, src/laser.cpp:29-99). */
+enum { LMC_SOURCE_POINT = 0, LMC_SOURCE_UNIFORM = 1, LMC_SOURCE_GAUSSIAN = 2 };
+enum {
+  LMC_TIME_DELTA = 0, LMC_TIME_GAUSSIAN = 1, LMC_TIME_TOPHAT = 2,
+  LMC_TIME_EXPONENTIAL = 3, LMC_TIME_PULSE_TRAIN = 4, LMC_TIME_CW = 5
+};
+typedef struct lmc_laser_desc {
+  double pos[3], dir[3], m[3], n[3];
+  double pol[4];        /* Em.re, Em.im, En.re, En.im                         */
+  double wavelength;    /* vacuum wavelength [mm]; k0 = 2*pi/wavelength        */
+  double sigma;
+  int32_t source;
+  int32_t temporal;
+  double pulse_duration, repetition_rate, time_offset; /* ps, Hz, ps        */
+} lmc_laser_desc;
+
+typedef struct lmc_scene_desc {
+  int32_t n_layers, n_species, n_sensors, reserved0;
+  const lmc_layer_desc* layers;
+  const lmc_species_desc* species;
+  const lmc_sensor_desc* sensors;
+  const double* tables;
+  int64_t n_tables;     /* doubles in `tables`                                */
+  int64_t n_out;        /* total accumulator doubles over all sensors         */
+  lmc_laser_desc laser;
+  double n_medium;      /* host index (Sample::refractive_index)              */
+} lmc_scene_desc;
+
+/* Precision / RNG modes. */
+enum { LMC_PREC_FP32 = 0, LMC_PREC_FP64 = 1 };
+enum {
+  LMC_RNG_PHILOX = 0,   /* Philox4x32-10 keyed by seed, counter by photon/event */
+  LMC_RNG_XOSHIRO = 1   /* xoshiro256++ re-seeded per photon with mix_seed, in
+                           the reference's draw order (src/simulation.cpp:197);
+                           FP64 only: trajectory-level comparison mode      */
+};
+
+/* SimConfig run controls (simulation.hpp:67-90) plus launch knobs. */
+typedef struct lmc_run_desc {
+  uint64_t n_photons;
+  uint64_t photon_offset;  /* first photon index (chunked / resumed runs)     */
+  uint64_t seed;
+  int32_t max_events;      /* SimConfig::MAX_EVENTS                           */
+  int32_t track_reverse_paths;
+  int32_t precision;       /* LMC_PREC_*                                      */
+  int32_t rng;             /* LMC_RNG_*                                       */
+  int32_t block_size;      /* threads per workgroup, 0 = default              */
+  int32_t blocks_per_cu;   /* persistent workgroups per CU, 0 = default       */
+  int32_t claim;           /* photons claimed per wave refill, 0 = default    */
+  int32_t reserved0;
+} lmc_run_desc;
+
+/* Exact device-side counters of the last run (feed the FLOP model). */
+typedef struct lmc_stats {
+  uint64_t photons;        /* photons launched                                */
+  uint64_t events;         /* scattering events                               */
+  uint64_t phi_trials;     /* azimuth rejection trials                        */
+  uint64_t crossings;      /* sensor-plane crossings processed                */
+  uint64_t cbs_detections; /* photons deposited with a reverse path (events>=2) */
+  double kernel_ms;        /* device time of all kernels of the run           */
+  double transport_ms;     /* device time of the transport kernel launches    */
+  double records_ms;       /* device time of the CBS reverse-path kernel      */
+  uint64_t transport_launches;
+} lmc_stats;
+
+/* Device facts used for launch shaping and roofline reporting. */
+typedef struct lmc_device_info {
+  int32_t n_cu;
+  int32_t clock_khz;
+  int64_t lds_per_block;
+  int64_t lds_per_cu;
+  int64_t global_mem;
+  char name[64];
+} lmc_device_info;
+int lmc_get_device_info(lmc_device_info* info);
+
+typedef struct lmc_scene lmc_scene;
+
+int lmc_abi_version(void);
+const char* lmc_last_error(void);
+int lmc_device_count(void);
+
+/* Upload a scene (tables converted to the run precision on device) and
+ * allocate its zeroed FP64 accumulators.
+ * Replaces: the per-thread SensorsGroup::clone (detector.cpp:299-307) and
+ * the scene construction reached from python_module.cpp:579-614. */
+int lmc_scene_create(const lmc_scene_desc* desc, lmc_scene** out);
+void lmc_scene_destroy(lmc_scene* s);
+
+/* Transport n_photons photons [photon_offset, photon_offset+n_photons) and
+ * ADD into the scene's device accumulators.
+ * Replaces: run_simulation_parallel(const SimConfig&) (simulation.cpp:105-249)
+ * with run_photon (:255-617), record_hit (detector.cpp:178-270) and the
+ * sensors' process_hit (:546-568, :780-841, :1231-1310). */
+int lmc_scene_run(lmc_scene* s, const lmc_run_desc* run, lmc_stats* stats);
+
+/* Add the device accumulators into host buffers (n doubles at `out`,
+ * layout of lmc_sensor_desc) and the per-sensor hit counters into `hits`
+ * (n_sensors entries); either may be NULL.
+ * Replaces: SensorsGroup::merge_from (detector.cpp:283-297). */
+int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits);
+/* Zero the device accumulators and hit counters. */
+int lmc_scene_zero(lmc_scene* s);
+
+/* ---- batched persistent launch over many independent scenes ----------- */
+typedef struct lmc_batch lmc_batch;
+/* All scenes must hold exactly one sensor of kind LMC_SENSOR_FARFIELD_CBS
+ * (plus optionally one LMC_SENSOR_REFLECTANCE). */
+int lmc_batch_create(int32_t n_scenes, const lmc_scene_desc* scenes, lmc_batch** out);
+void lmc_batch_destroy(lmc_batch* b);
+/* Run budgets[i] photons of scene i (photon indices from 0) in ONE launch;
+ * if use_graph != 0 the zero+transport sequence is captured once into a
+ * hipGraph and replayed.  Adds into the per-scene device accumulators. */
+int lmc_batch_run(lmc_batch* b, const uint64_t* budgets, const lmc_run_desc* run,
+                  int32_t use_graph, lmc_stats* stats);
+int lmc_batch_read(lmc_batch* b, int32_t scene, double* out, int64_t n, uint64_t* hits);
+int lmc_batch_zero(lmc_batch* b);
+
+/* ---- adjoint (path-length / event-count score) ------------------------ */
+/* Forward transport that additionally accumulates, for every accumulator
+ * double d of the scene and every parameter p, score_p * contribution_d,
+ * with per-layer parameters ordered [mu_s(l), mu_a(l), g(l)] for l in layers
+ * (3*n_layers parameters; g rows are zero for non-HG layers).
+ * `grad` receives (added) 3*n_layers*n_out doubles, [p][d]. The ordinary
+ * accumulators are filled as by lmc_scene_run.  The derivative assumes each
+ * layer's mean free path tracks 1/mu_t (the descriptor's mfp must equal
+ * 1/mu_t for every layer). */
+int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* run, double* grad,
+                          int64_t n_grad, lmc_stats* stats);
+
+/* ---- host-side table builders (src/mie/, src/table.cpp, src/phase.cpp) --- */
+/* Mie amplitudes at n_mu cosines for the refractive index m_re + i*m_im,
+ * with MIEV0's sign conventions (src/mie/dmiev.cpp:49-62,
+ * src/mie/MIEV0noP.f:160-161,472-485).  s1/s2: 2*n_mu doubles (re, im). */
+int lmc_mie_amplitudes(double x, double m_re, double m_im, int32_t n_mu,
+                       const double* mu, double* s1, double* s2);
+/* Q_ext, Q_sca, g (src/mie/dmiev.cpp:65-81). */
+int lmc_mie_info(double x, double m_re, double m_im, double* qext, double* qsca,
+                 double* g);
+/* Normalised SamplingTable CDF from pdf values at nDiv+1 nodes
+ * (src/table.cpp:6-48). */
+int lmc_sampling_cdf(int32_t n_div, double min_val, double max_val,
+                     const double* pdf, double* cdf);
+
+/* ---- known-answer probes (tests only; no counterpart in the reference) -- */
+/* Philox4x32-10 on device: out[4*i..] = philox(ctr[4*i..], key). */
+int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
+                     uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUMINIS_HIP_H */

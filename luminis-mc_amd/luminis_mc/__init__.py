@@ -1,0 +1,27 @@
+"""luminis_mc -- MI355X (gfx950) implementation of luminis-mc's photon-transport path.
+
+Same public names as the reference package's transport surface
+(python/luminis_mc/__init__.py:1-76): build a Sample, a Laser and a
+SensorsGroup, set up a SimConfig, call run_simulation_parallel(config) and
+post-process the sensors with postprocess_*.  The transport runs as HIP
+kernels on the GPU through libluminis_hip.so.
+"""
+from ._lib import lib as _load_library
+from .linalg import (CMatrix, CVec2, Matrix, Rng, Vec2, Vec3, calculate_rotation_angle, cross, dot, matcmul,
+                     matcmulscalar, mix_seed, norm)
+from .phase import (DrainePhaseFunction, HenyeyGreensteinPhaseFunction, MiePhaseFunction, PhaseFunction,
+                    RayleighDebyeEMCPhaseFunction, RayleighDebyePhaseFunction, RayleighPhaseFunction,
+                    SamplingTable, UniformPhaseFunction, form_factor, mie_amplitudes, mie_info)
+from .medium import Layer, MieMedium, MixtureLayer, RGDMedium, Sample, SampleLayer, ScatteringMedium
+from .sensors import (CrossingDirection, FarFieldCBSSensor, FarFieldFluenceSensor, Laser, LaserSource,
+                      PlanarFieldSensor, PlanarFluenceSensor, ReflectanceSensor, Sensor, SensorsGroup,
+                      TemporalProfile)
+from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFieldProcessed,
+                      PlanarFluenceProcessed, StokesMatrixProcessed, StokesRadialProcessed,
+                      postprocess_farfield_cbs, postprocess_planar_field, postprocess_planar_fluence,
+                      solid_angle_bin)
+from .simulation import GpuScene, SimConfig, device_info, run_simulation_parallel, validate_sim_config
+
+_load_library()
+
+__all__ = [n for n in dir() if not n.startswith("_")]
