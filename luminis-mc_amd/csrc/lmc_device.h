@@ -272,7 +272,7 @@ struct SensorDev {
 
 struct PlaneDev {
   double z;
-  int32_t first, count;
+  int32_t first, count, has_cbs, mult;  // mult: crossing records per photon, worst case
 };
 
 }  // namespace lmc

@@ -1,0 +1,1083 @@
+// Device code of the gfx950 transport path: kernel parameters, per-lane
+// photon state, the per-event body, sensor deposits and the kernels.
+//
+// Register budget.  A photon's live state is what the reference keeps in
+// Photon (photon.hpp:65-243) minus what is constant or dead on this path:
+// the launch frame P0 and Jones vector E0 are scene constants (scalar
+// registers), the exit frame P_n is the current frame, Pn2 and the
+// normalised T product never reach a detector, and the frames after the
+// first scatter and before the last one keep only their m and s rows
+// (n = s x m for the right-handed frames).  Event counters are wave-uniform
+// (scalar registers); crossing counters go to LDS.
+#pragma once
+
+#include "lmc_device.h"
+
+namespace lmc {
+
+template <class R> struct KParams {
+  const LayerDev<R>* layers;
+  const SpeciesDev<R>* species;
+  const R* cdf;
+  const uint16_t* guide;
+  const STab<R>* stab;
+  const R* sel;
+  const SensorDev* sensors;
+  const PlaneDev* planes;
+  double* out;
+  unsigned long long* hits;
+  unsigned long long* counter;
+  unsigned long long* stats;  // photons, events, trials, crossings, cbs
+  double* grad;
+  R* qf;
+  double* qd;                   // FP64 optical path at the crossing
+  int32_t* qi;
+  unsigned long long* q_count;  // [0] records written, [1] overflowed
+  int64_t q_cap;
+  int64_t n_out;
+  uint64_t n_photons, offset;
+  int32_t n_layers, n_species, n_sensors, n_planes;
+  int32_t cdf_total, guide_total, stab_total, sel_total;
+  int32_t lds_grid, launch_layer, source, temporal;
+  int32_t max_events, claim, top_finite;
+  uint32_t key0, key1, idx_hi;
+  R z_top;
+  // laser
+  double lpos_x, lpos_y, sigma, pulse, rep, t_off;
+  R lpos_z;
+  R ldir[3], lm[3], ln[3];
+  R pol[4];
+  R k0;
+};
+
+// One queued plane crossing accepted by at least one sensor of its plane:
+// everything the sensors' process_hit needs (detector.cpp:178-270 builds the
+// same InteractionInfo), so the transport kernel carries none of the sensor
+// code.  The CBS history fields (frames keep their m and s rows) are written
+// only for photons with >= 2 events at a plane holding a CBS sensor, the
+// scores only in adjoint runs.  Structure of arrays, field-major, stride q_cap.
+enum RecF {
+  F_HX = 0, F_HY = 1, F_ARR = 2, F_W = 3, F_EM = 4, F_P = 8,  // light part
+  F_M1 = 17, F_S1 = 20, F_MQ = 23, F_SQ = 26, F_T = 29, F_D = 37,  // CBS history
+  F_SCORE = 40, kRecF = 52
+};
+enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, kRecI = 4 };
+
+template <class R> struct SceneView {
+  const LayerDev<R>* L;
+  const SpeciesDev<R>* S;
+  const R* cdf;
+  const uint16_t* guide;
+  const STab<R>* stab;
+  const R* sel;
+  const SensorDev* sens;
+  const PlaneDev* planes;
+  double* lgrid;     // LDS accumulators
+  unsigned* lhits;   // LDS hit counters [kMaxSensors], then crossings, cbs
+};
+
+// xoshiro state exists only in the sequential-stream build.
+template <bool kXo> struct XoState {
+  __device__ __forceinline__ void init(uint64_t, uint64_t) {}
+};
+template <> struct XoState<true> {
+  XoshiroStream s;
+  __device__ __forceinline__ void init(uint64_t seed, uint64_t photon) { s.init(seed, 0, photon); }
+};
+
+template <class R, bool kXo> struct Photon {
+  V3<R> pos;
+  Frame<R> P;
+  cx<R> Em, En;
+  R w, gpl, t0;
+  double opl;
+  int32_t events, layer;
+  uint32_t idx;  // photon index, low word (the launch's high word is uniform)
+  bool alive;
+  // CBS history
+  V3<R> m1, s1, mq, sq;  // frame after the first scatter, frame before the last one
+  V3<R> r1, rn;
+  cx<R> T[4];            // raw interior product J_{n-1}...J_2 (Frobenius-normalised)
+  cx<R> b2, b1;          // last event's J_n = diag(b2, b1) R(bc, bs), pending
+  R bc, bs;
+  int32_t sp;            // (first species << 16) | last species
+  // Philox draws of the current event kept past the flight
+  R u_sp, u_th, u_rr;
+  XoState<kXo> xo;
+  R aL[kMaxGradLayers], aN[kMaxGradLayers], aG[kMaxGradLayers];
+};
+
+template <class R> __device__ __forceinline__ V3<R> cross3(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t key1, uint32_t idx, uint32_t idx_hi,
+                                             uint32_t ev, uint32_t sub) {
+  return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), key0, key1);
+}
+
+// Inverse-CDF lookup: SamplingTable::Sample (src/table.cpp:50-66).  The
+// lower_bound is narrowed by a guide table g[j] = lower_bound(j/G), so the
+// result is exactly the reference's index on the same CDF values.
+template <class R>
+__device__ __forceinline__ R table_sample(const SceneView<R>& v, const SpeciesDev<R>& s, R u) {
+  const R* c = v.cdf + s.cdf_off;
+  const uint16_t* g = v.guide + s.guide_off;
+  int j = static_cast<int>(u * static_cast<R>(s.guide_n));
+  j = j < 0 ? 0 : (j >= s.guide_n ? s.guide_n - 1 : j);
+  int lo = g[j], hi = g[j + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (c[mid] < u)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo == 0) return s.cdf_min;
+  const R ca = c[lo - 1], cb = c[lo];
+  const R xa = s.cdf_min + static_cast<R>(lo - 1) * s.cdf_step;
+  const R t = (u - ca) / (cb - ca);
+  return xa + t * s.cdf_step;
+}
+
+// DataTable::Sample on the uniform theta grid (src/table.cpp:100-118).
+template <class R>
+__device__ __forceinline__ void s_lookup(const STab<R>* stab, const SpeciesDev<R>& s, R th, cx<R>& s1, cx<R>& s2) {
+  const STab<R>* tb = stab + s.s_off;
+  const R pos = th * s.inv_ds;
+  const int last = s.s_n - 1;
+  if (!(pos > R(0))) {  // also maps a NaN angle to node 0: indices stay in bounds
+    stab_get(tb[0], s1, s2);
+    return;
+  }
+  if (pos >= static_cast<R>(last)) {
+    stab_get(tb[last], s1, s2);
+    return;
+  }
+  const int i = static_cast<int>(pos);
+  const R t = pos - static_cast<R>(i);
+  cx<R> a1, a2, b1, b2;
+  stab_get(tb[i], a1, a2);
+  stab_get(tb[i + 1], b1, b2);
+  s1 = {a1.r + t * (b1.r - a1.r), a1.i + t * (b1.i - a1.i)};
+  s2 = {a2.r + t * (b2.r - a2.r), a2.i + t * (b2.i - a2.i)};
+}
+
+template <class R> __device__ __forceinline__ R clamp1(R x) { return x > R(1) ? R(1) : (x < R(-1) ? R(-1) : x); }
+
+// Exit polar angle from the -z axis.  FP64: acos(-s_z) as the reference
+// (detector.cpp:1274); FP32: atan2(|s_perp|, -s_z), which keeps ~1e-7 rad
+// resolution near exact backscatter where acos in FP32 would quantise to 3e-4.
+template <class R> __device__ __forceinline__ R exit_theta(V3<R> s) {
+  if constexpr (sizeof(R) == 8) {
+    return M<R>::acos(-s.z);
+  } else {
+    return M<R>::atan2(M<R>::sqrt(s.x * s.x + s.y * s.y), -s.z);
+  }
+}
+template <class R> __device__ __forceinline__ R exit_phi(V3<R> s) {
+  R phi = M<R>::atan2(s.y, s.x);
+  if (phi < R(0)) phi += R(2.0 * 3.14159265358979323846);
+  return phi;
+}
+
+// Reverse-path Jones vector: Xu (2008) three-stage construction,
+// reverse_field_prefix/suffix (detector.cpp:1080-1161).  P0/E0 are the
+// launch frame and Jones vector; Pq is the frame before the last scatter
+// (P_{n-1}), P1 the frame after the first scatter, Pn the exit frame and T the
+// raw interior product J_{n-1}...J_2.  Returns the normalised reverse field in
+// the exit basis (m_n, n_n).
+template <class R>
+__device__ __forceinline__ void reverse_field(const KParams<R>& p, const STab<R>* stab, const SpeciesDev<R>& spn,
+                                              const SpeciesDev<R>& sp1, const Frame<R>& P1, const Frame<R>& Pq,
+                                              const Frame<R>& Pn, const cx<R> T[4], cx<R>& Erm, cx<R>& Ern) {
+  const V3<R> s0{p.ldir[0], p.ldir[1], p.ldir[2]};
+  const V3<R> m0{p.lm[0], p.lm[1], p.lm[2]};
+  const V3<R> n0{p.ln[0], p.ln[1], p.ln[2]};
+  // Stage A: scatter at r_n, s0 -> -s_{n-1}
+  const V3<R> s_out_a{-Pq.s.x, -Pq.s.y, -Pq.s.z};
+  const R th_a = M<R>::acos(clamp1(dot(s0, s_out_a)));
+  cx<R> a1, a2;
+  s_lookup(stab, spn, th_a, a1, a2);
+  const V3<R> np = safe_unit(cross_unit(s0, s_out_a), n0);
+  const V3<R> mp_in = safe_unit(cross_unit(np, s0), m0);
+  const V3<R> mp_out = safe_unit(cross_unit(np, s_out_a), Pq.m);
+  const cx<R> E0m{p.pol[0], p.pol[1]}, E0n{p.pol[2], p.pol[3]};
+  // E = S_a * rot2(mp_in, np, m0, n0) * E0
+  R r00 = dot(mp_in, m0), r01 = dot(mp_in, n0), r10 = dot(np, m0), r11 = dot(np, n0);
+  cx<R> xm = cadd(cscale(E0m, r00), cscale(E0n, r01));
+  cx<R> xn = cadd(cscale(E0m, r10), cscale(E0n, r11));
+  xm = cmul(a2, xm);
+  xn = cmul(a1, xn);
+  // rot2(m_{n-1}, -n_{n-1}, mp_out, np)
+  const V3<R> nq{-Pq.n.x, -Pq.n.y, -Pq.n.z};
+  r00 = dot(Pq.m, mp_out);
+  r01 = dot(Pq.m, np);
+  r10 = dot(nq, mp_out);
+  r11 = dot(nq, np);
+  cx<R> ym = cadd(cscale(xm, r00), cscale(xn, r01));
+  cx<R> yn = cadd(cscale(xm, r10), cscale(xn, r11));
+  // Stage B: Q T^T Q with Q = diag(1,-1): [[T00, -T10], [-T01, T11]]
+  xm = cadd(cmul(T[0], ym), cscale(cmul(T[2], yn), R(-1)));
+  xn = cadd(cscale(cmul(T[1], ym), R(-1)), cmul(T[3], yn));
+  // Stage C: scatter at r_1, -s1 -> s_n
+  const V3<R> s_in_c{-P1.s.x, -P1.s.y, -P1.s.z};
+  const R th_c = M<R>::acos(clamp1(dot(s_in_c, Pn.s)));
+  cx<R> c1, c2;
+  s_lookup(stab, sp1, th_c, c1, c2);
+  const V3<R> n1neg{-P1.n.x, -P1.n.y, -P1.n.z};
+  const V3<R> npp = safe_unit(cross_unit(s_in_c, Pn.s), n1neg);
+  const V3<R> mpp_in = safe_unit(cross_unit(npp, s_in_c), P1.m);
+  const V3<R> mpp_out = safe_unit(cross_unit(npp, Pn.s), Pn.m);
+  r00 = dot(mpp_in, P1.m);
+  r01 = dot(mpp_in, n1neg);
+  r10 = dot(npp, P1.m);
+  r11 = dot(npp, n1neg);
+  ym = cadd(cscale(xm, r00), cscale(xn, r01));
+  yn = cadd(cscale(xm, r10), cscale(xn, r11));
+  ym = cmul(c2, ym);
+  yn = cmul(c1, yn);
+  r00 = dot(Pn.m, mpp_out);
+  r01 = dot(Pn.m, npp);
+  r10 = dot(Pn.n, mpp_out);
+  r11 = dot(Pn.n, npp);
+  xm = cadd(cscale(ym, r00), cscale(yn, r01));
+  xn = cadd(cscale(ym, r10), cscale(yn, r11));
+  const R nrm2 = cnorm(xm) + cnorm(xn);
+  if (nrm2 > R(0) && M<R>::sqrt(nrm2) > static_cast<R>(sizeof(R) == 8 ? 1e-300 : 1e-30)) {
+    const R inv = R(1) / M<R>::sqrt(nrm2);
+    Erm = cscale(xm, inv);
+    Ern = cscale(xn, inv);
+  } else {
+    Erm = {R(0), R(0)};
+    Ern = {R(0), R(0)};
+  }
+}
+
+// Accumulate one value: LDS copy (ds_add_f64) for privatised grids, HBM
+// (global_atomic_add_f64) otherwise.  The two pointers keep their address
+// spaces so neither path degrades to a flat atomic.
+typedef __attribute__((address_space(3))) double lds_double;
+__device__ __forceinline__ void acc_add(double* lds_base, double* hbm_base, bool lds, int64_t off, double val) {
+  if (val != 0.0) {
+    if (lds)
+      __hip_atomic_fetch_add((lds_double*)(lds_base + off), val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      unsafeAtomicAdd(hbm_base + off, val);
+  }
+}
+
+// Adjoint scores of one detected photon for every layer parameter:
+// d ln(path density x weight)/d mu_s = N_l/mu_s - L_l, d/d mu_a = -L_l,
+// d/dg = sum of d ln p_HG/dg over the layer's scatters.
+template <class R, bool kXo>
+__device__ __forceinline__ void adj_scores(const KParams<R>& p, const SceneView<R>& v, const Photon<R, kXo>& ph,
+                                           R corr, R score[3 * kMaxGradLayers]) {
+#pragma unroll
+  for (int l = 0; l < kMaxGradLayers; ++l) {
+    const R mus = l < p.n_layers ? v.L[l].mu_s : R(1);
+    const R Ll = ph.aL[l] - (l == ph.layer ? corr : R(0));
+    score[3 * l + 0] = (mus > R(0) ? ph.aN[l] / mus : R(0)) - Ll;
+    score[3 * l + 1] = -Ll;
+    score[3 * l + 2] = ph.aG[l];
+  }
+}
+
+template <class R, bool kAdj>
+__device__ __forceinline__ void deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int64_t off,
+                                        double val, const R* score) {
+  acc_add(lds_grid + sd.lds_off, p.out + sd.out_off, sd.lds != 0, off, val);
+  if constexpr (kAdj) {
+    const int np = 3 * (p.n_layers < kMaxGradLayers ? p.n_layers : kMaxGradLayers);
+    for (int q = 0; q < np; ++q)
+      if (score[q] != R(0)) unsafeAtomicAdd(p.grad + q * p.n_out + sd.out_off + off, val * static_cast<double>(score[q]));
+  }
+}
+
+// Stokes S0..S3 of a lab-frame field (Ex, Ey): |Ex|^2 + |Ey|^2, |Ex|^2 - |Ey|^2,
+// 2 Re(Ex Ey*), 2 Im(Ex Ey*) (detector.cpp:823-827).
+template <class R> __device__ __forceinline__ R stokes(int q, cx<R> Ex, cx<R> Ey) {
+  if (q == 0) return cnorm(Ex) + cnorm(Ey);
+  if (q == 1) return cnorm(Ex) - cnorm(Ey);
+  if (q == 2) return R(2) * (Ex.r * Ey.r + Ex.i * Ey.i);
+  return R(2) * (Ex.i * Ey.r - Ex.r * Ey.i);
+}
+
+// Deposit the 8 CBS grids for one detection: coherent |E_f + E_r|^2 terms and
+// incoherent |E_f|^2 + |E_r|^2 terms (accumulate_stokes, detector.cpp:1194-1226).
+template <class R, bool kAdj>
+__device__ __forceinline__ void cbs_deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int64_t bin,
+                                            int t_idx, cx<R> Efx, cx<R> Efy, cx<R> Erx, cx<R> Ery, const R* score) {
+  const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
+  const int64_t plane = tplane * sd.n2;
+  const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
+#pragma unroll 1
+  for (int q = 0; q < 8; ++q) {
+    const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
+    const double dv = static_cast<double>(val);
+    deposit<R, kAdj>(p, sd, lds_grid, q * plane + bin, dv, score);
+    if (t_idx >= 1) deposit<R, kAdj>(p, sd, lds_grid, q * plane + t_idx * tplane + bin, dv, score);
+  }
+}
+
+// Unit right-handed frame rows m, n = s x m, s from stored m and s rows.
+template <class R> __device__ __forceinline__ Frame<R> frame_of(V3<R> m, V3<R> s) {
+  Frame<R> f;
+  f.m = m;
+  f.s = s;
+  f.n = cross3(s, m);
+  return f;
+}
+
+// Sensor::check_conditions (detector.cpp:374-417).
+template <class R>
+__device__ __forceinline__ bool accepts(const SensorDev& sd, V3<R> hp, V3<R> dir, int cdir, int events) {
+  if (sd.f_dir_en && sd.f_dir != LMC_DIR_BOTH && sd.f_dir != cdir) return false;
+  if (sd.f_ev_en && (events < sd.f_ev_min || events > sd.f_ev_max)) return false;
+  if (sd.f_th_en) {
+    const R ct = -dir.z;
+    if (ct < static_cast<R>(sd.cos_th_min) || ct > static_cast<R>(sd.cos_th_max)) return false;
+  }
+  if (sd.f_ph_en) {
+    const R phi = exit_phi(dir);
+    if (phi < static_cast<R>(sd.ph_min) || phi > static_cast<R>(sd.ph_max)) return false;
+  }
+  if (sd.f_pos_en) {
+    if (hp.x < static_cast<R>(sd.x_min) || hp.x > static_cast<R>(sd.x_max)) return false;
+    if (hp.y < static_cast<R>(sd.y_min) || hp.y > static_cast<R>(sd.y_max)) return false;
+  }
+  return true;
+}
+
+// A crossing of plane `pl` by the segment prev -> pos: hit point and the
+// overshoot-corrected paths (detector.cpp:194-235); the sensors' filters
+// decide acceptance and absorption here, the deposits happen in
+// k_crossings.  Returns true when an absorbing sensor accepted the photon.
+template <class R, bool kXo, bool kAdj>
+__device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
+                                          V3<R> prev, int cdir) {
+  const R zp = static_cast<R>(v.planes[pl].z);
+  const V3<R> d{ph.pos.x - prev.x, ph.pos.y - prev.y, ph.pos.z - prev.z};
+  const R t = (zp - prev.z) / d.z;
+  const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
+  const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
+  const R corr = M<R>::sqrt(dot(ov, ov));
+  atomicAdd(&v.lhits[kMaxSensors], 1u);
+  bool killed = false;
+  int mask = 0;
+  const int first = v.planes[pl].first, cnt = v.planes[pl].count;
+  for (int k = 0; k < cnt; ++k) {
+    const SensorDev& sd = v.sens[first + k];
+    if (!accepts(sd, hp, ph.P.s, cdir, ph.events)) continue;
+    mask |= 1 << k;
+    if (sd.absorb) killed = true;
+  }
+  if (mask == 0) return killed;
+  const unsigned long long slot = atomicAdd(&p.q_count[0], 1ull);
+  if (static_cast<int64_t>(slot) >= p.q_cap) {
+    atomicAdd(&p.q_count[1], 1ull);
+    return killed;
+  }
+  const int64_t c = p.q_cap;
+  R* f = p.qf + slot;
+  const R gpl_hit = corr > R(0) ? ph.gpl - corr : ph.gpl;
+  double opl_hit = ph.opl;
+  if (corr > R(0)) opl_hit -= static_cast<double>(corr) * static_cast<double>(v.L[ph.layer].n);
+  f[F_HX * c] = hp.x;
+  f[F_HY * c] = hp.y;
+  f[F_ARR * c] = ph.t0 + gpl_hit;
+  f[F_W * c] = ph.w;
+  f[(F_EM + 0) * c] = ph.Em.r;
+  f[(F_EM + 1) * c] = ph.Em.i;
+  f[(F_EM + 2) * c] = ph.En.r;
+  f[(F_EM + 3) * c] = ph.En.i;
+  {
+    const R fr[9] = {ph.P.m.x, ph.P.m.y, ph.P.m.z, ph.P.n.x, ph.P.n.y, ph.P.n.z, ph.P.s.x, ph.P.s.y, ph.P.s.z};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f[(F_P + k) * c] = fr[k];
+  }
+  p.qd[slot] = opl_hit;
+  if (ph.events >= 2 && v.planes[pl].has_cbs) {
+    const R h[23] = {ph.m1.x, ph.m1.y, ph.m1.z, ph.s1.x, ph.s1.y, ph.s1.z, ph.mq.x, ph.mq.y,
+                     ph.mq.z, ph.sq.x, ph.sq.y, ph.sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r, ph.T[1].i,
+                     ph.T[2].r, ph.T[2].i, ph.T[3].r, ph.T[3].i, ph.rn.x - ph.r1.x, ph.rn.y - ph.r1.y,
+                     ph.rn.z - ph.r1.z};
+#pragma unroll
+    for (int k = 0; k < 23; ++k) f[(F_M1 + k) * c] = h[k];
+  }
+  if constexpr (kAdj) {
+    R score[3 * kMaxGradLayers];
+    adj_scores(p, v, ph, corr, score);
+#pragma unroll
+    for (int q = 0; q < 3 * kMaxGradLayers; ++q) f[(F_SCORE + q) * c] = score[q];
+  }
+  int32_t* iq = p.qi + slot;
+  iq[I_PLANE * c] = pl;
+  iq[I_MASK * c] = mask;
+  iq[I_EVENTS * c] = ph.events;
+  iq[I_SP * c] = ph.sp;
+  return killed;
+}
+
+// SensorsGroup::record_hit (detector.cpp:178-270) for the segment prev->pos:
+// planes within [min z, max z] in increasing z.
+template <class R, bool kXo, bool kAdj>
+__device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, V3<R> prev) {
+  const R z1 = prev.z, z2 = ph.pos.z;
+  if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
+  const R zmin = z1 < z2 ? z1 : z2, zmax = z1 < z2 ? z2 : z1;
+  bool killed = false;
+  for (int pl = 0; pl < p.n_planes; ++pl) {
+    const R zp = static_cast<R>(v.planes[pl].z);
+    if (zp >= zmin && zp <= zmax)
+      killed |= plane_hit<R, kXo, kAdj>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
+  }
+  return killed;
+}
+
+// Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
+// The Philox stream draws emission uniforms from event slot 0xFFFFFFFF.
+template <class R, bool kXo>
+__device__ __forceinline__ void launch(const KParams<R>& p, Photon<R, kXo>& ph, uint32_t idx) {
+  ph.idx = idx;
+  R x = static_cast<R>(p.lpos_x), y = static_cast<R>(p.lpos_y);
+  R t0 = static_cast<R>(p.t_off * 1e-3);
+  const R pulse = static_cast<R>(p.pulse * 1e-3);
+  const R fwhm_to_sigma = static_cast<R>(1.0 / (2.0 * 1.1774100225154747));  // 1/(2 sqrt(2 ln 2))
+  if constexpr (kXo) {
+    ph.xo.init((static_cast<uint64_t>(p.key1) << 32) | p.key0, (static_cast<uint64_t>(p.idx_hi) << 32) | idx);
+    double xd = p.lpos_x, yd = p.lpos_y, td = p.t_off * 1e-3;
+    const double kTwoPi = 2.0 * 3.14159265358979323846;
+    if (p.source == LMC_SOURCE_UNIFORM) {
+      const double th = ph.xo.s.uniform() * kTwoPi;
+      const double r = ::sqrt(ph.xo.s.uniform()) * p.sigma;
+      xd += r * ::cos(th);
+      yd += r * ::sin(th);
+    } else if (p.source == LMC_SOURCE_GAUSSIAN) {
+      xd = ph.xo.s.normal(xd, p.sigma);
+      yd = ph.xo.s.normal(yd, p.sigma);
+    }
+    switch (p.temporal) {
+      case LMC_TIME_GAUSSIAN: td = ph.xo.s.normal(td, p.pulse * 1e-3 / (2.0 * 1.1774100225154747)); break;
+      case LMC_TIME_TOPHAT: td = td + (ph.xo.s.uniform() - 0.5) * p.pulse * 1e-3; break;
+      case LMC_TIME_EXPONENTIAL: td = td - ::log(ph.xo.s.uniform()) * p.pulse * 1e-3; break;
+      case LMC_TIME_PULSE_TRAIN: if (p.rep > 0.0) td = td + ph.xo.s.uniform() * (1e9 / p.rep); break;
+      case LMC_TIME_CW: if (p.pulse > 0.0) td = td + ph.xo.s.uniform() * p.pulse * 1e-3; break;
+      default: break;
+    }
+    x = static_cast<R>(xd);
+    y = static_cast<R>(yd);
+    t0 = static_cast<R>(td);
+  } else {
+    if (p.source != LMC_SOURCE_POINT) {
+      const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 0u);
+      const R sg = static_cast<R>(p.sigma);
+      if (p.source == LMC_SOURCE_UNIFORM) {
+        R sn, cs;
+        M<R>::sincos_rev(u01<R>(b.x), &sn, &cs);
+        const R r = M<R>::sqrt(u01<R>(b.y)) * sg;
+        x += r * cs;
+        y += r * sn;
+      } else {  // Box-Muller, cosine branch for each axis (rng.cpp:18-28)
+        R sn, cs;
+        M<R>::sincos_rev(u01<R>(b.y), &sn, &cs);
+        x += M<R>::sqrt(R(-2) * M<R>::log(u01<R>(b.x))) * cs * sg;
+        M<R>::sincos_rev(u01<R>(b.w), &sn, &cs);
+        y += M<R>::sqrt(R(-2) * M<R>::log(u01<R>(b.z))) * cs * sg;
+      }
+    }
+    if (p.temporal != LMC_TIME_DELTA) {
+      const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 1u);
+      const R ua = u01<R>(b.x);
+      switch (p.temporal) {
+        case LMC_TIME_GAUSSIAN: {
+          R sn, cs;
+          M<R>::sincos_rev(u01<R>(b.y), &sn, &cs);
+          t0 += M<R>::sqrt(R(-2) * M<R>::log(ua)) * cs * pulse * fwhm_to_sigma;
+          break;
+        }
+        case LMC_TIME_TOPHAT: t0 += (ua - R(0.5)) * pulse; break;
+        case LMC_TIME_EXPONENTIAL: t0 -= M<R>::log(ua) * pulse; break;
+        case LMC_TIME_PULSE_TRAIN: if (p.rep > 0.0) t0 += ua * static_cast<R>(1e9 / p.rep); break;
+        case LMC_TIME_CW: if (p.pulse > 0.0) t0 += ua * pulse; break;
+        default: break;
+      }
+    }
+  }
+  ph.pos = {x, y, p.lpos_z};
+  ph.P.m = {p.lm[0], p.lm[1], p.lm[2]};
+  ph.P.n = {p.ln[0], p.ln[1], p.ln[2]};
+  ph.P.s = {p.ldir[0], p.ldir[1], p.ldir[2]};
+  ph.Em = {p.pol[0], p.pol[1]};
+  ph.En = {p.pol[2], p.pol[3]};
+  ph.w = R(1);
+  ph.gpl = R(0);
+  ph.t0 = t0;
+  ph.opl = 0.0;
+  ph.events = 0;
+  ph.layer = p.launch_layer;
+  ph.alive = true;
+  ph.m1 = ph.P.m;
+  ph.s1 = ph.P.s;
+  ph.mq = ph.P.m;
+  ph.sq = ph.P.s;
+  ph.r1 = ph.pos;
+  ph.rn = ph.pos;
+  ph.T[0] = {R(1), R(0)};
+  ph.T[1] = {R(0), R(0)};
+  ph.T[2] = {R(0), R(0)};
+  ph.T[3] = {R(1), R(0)};
+  ph.b2 = {R(1), R(0)};
+  ph.b1 = {R(1), R(0)};
+  ph.bc = R(1);
+  ph.bs = R(0);
+  ph.sp = 0;
+#pragma unroll
+  for (int l = 0; l < kMaxGradLayers; ++l) {
+    ph.aL[l] = R(0);
+    ph.aN[l] = R(0);
+    ph.aG[l] = R(0);
+  }
+}
+
+template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path(Photon<R, kXo>& ph, R d) {
+  if constexpr (kAdj) {
+#pragma unroll
+    for (int l = 0; l < kMaxGradLayers; ++l)
+      if (l == ph.layer) ph.aL[l] += d;
+  }
+}
+
+// One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
+// number of azimuth trials used.
+template <class R, bool kXo, bool kTrack, bool kAdj>
+__device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
+  R u_fp;
+  if constexpr (kXo) {
+    u_fp = static_cast<R>(ph.xo.s.uniform());
+  } else {
+    const uint4 b = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), 0u);
+    u_fp = u01<R>(b.x);
+    ph.u_sp = u01<R>(b.y);
+    ph.u_th = u01<R>(b.z);
+    ph.u_rr = u01<R>(b.w);
+  }
+  R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
+  const V3<R> dir = ph.P.s;
+  R new_z = ph.pos.z + dir.z * step;
+  // interface crossings (simulation.cpp:317-383), with the layer index tracked
+  // explicitly (the reference nudges z by 1e-12 and re-derives it)
+  for (;;) {
+    R zi = R(0);
+    bool has = false;
+    if (new_z > ph.pos.z) {
+      if (ph.layer + 1 < p.n_layers) {
+        zi = v.L[ph.layer].z_max;
+        has = zi <= new_z;
+      }
+    } else if (new_z < ph.pos.z) {
+      if (ph.layer > 0) {
+        zi = v.L[ph.layer].z_min;
+        has = zi >= new_z;
+      }
+    }
+    if (!has) break;
+    const R dzt = dir.z * step;
+    if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) break;
+    const R t = (zi - ph.pos.z) / dzt;
+    if (t < R(0) || t > R(1)) break;
+    const R step_to = step * t;
+    const R rem = (step - step_to) * v.L[ph.layer].mu_t;
+    ph.gpl += step_to;
+    ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step_to);
+    add_path<R, kXo, kAdj>(ph, step_to);
+    const V3<R> prev = ph.pos;
+    ph.pos.x += dir.x * step_to;
+    ph.pos.y += dir.y * step_to;
+    const bool up = dir.z > R(0);
+    if constexpr (sizeof(R) == 8)
+      ph.pos.z = zi + (up ? 1e-12 : -1e-12);
+    else
+      ph.pos.z = zi;
+    if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+      ph.alive = false;
+      return 0;
+    }
+    ph.layer += up ? 1 : -1;
+    const R mt = v.L[ph.layer].mu_t;
+    step = mt > R(0) ? rem / mt : rem;
+    new_z = ph.pos.z + dir.z * step;
+  }
+  ph.gpl += step;
+  ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step);
+  add_path<R, kXo, kAdj>(ph, step);
+  {
+    const V3<R> prev = ph.pos;
+    ph.pos.x += dir.x * step;
+    ph.pos.y += dir.y * step;
+    ph.pos.z += dir.z * step;
+    if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+      ph.alive = false;
+      return 0;
+    }
+  }
+  // Sample::is_inside (sample.cpp:268-288)
+  if (!(ph.pos.z >= R(0)) || (p.top_finite && ph.pos.z >= p.z_top) || !isfinite(ph.pos.x) ||
+      !isfinite(ph.pos.y) || !isfinite(ph.pos.z)) {
+    ph.alive = false;
+    return 0;
+  }
+  // species of this vertex (MixtureLayer::select_scatter_medium, sample.cpp:96-104)
+  const LayerDev<R>& Ld = v.L[ph.layer];
+  int sp = Ld.sp_first;
+  if (Ld.sp_count > 1) {
+    R u;
+    if constexpr (kXo)
+      u = static_cast<R>(ph.xo.s.uniform());
+    else
+      u = ph.u_sp;
+    const R* c = v.sel + Ld.sel_off;
+    int lo = 0, hi = Ld.sp_count;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c[mid] < u)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    sp += lo < Ld.sp_count ? lo : Ld.sp_count - 1;
+  }
+  const SpeciesDev<R>& S = v.S[sp];
+  // theta (medium.cpp:51-58 -> phase.cpp)
+  R uth;
+  if constexpr (kXo)
+    uth = static_cast<R>(ph.xo.s.uniform());
+  else
+    uth = ph.u_th;
+  R theta;
+  const int kind = S.kind;
+  if (kind == LMC_PHASE_HG) {  // phase.cpp:108-115
+    const R g = S.hg_g, gg = g * g;
+    const R frac = (R(1) - gg) / (R(1) - g + R(2) * g * uth);
+    R c = (R(1) + gg - frac * frac) / (R(2) * g);
+    if constexpr (sizeof(R) == 4) c = clamp1(c);
+    theta = M<R>::acos(c);
+    if constexpr (kAdj) {
+#pragma unroll
+      for (int l = 0; l < kMaxGradLayers; ++l)
+        if (l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - c) / (R(1) + gg - R(2) * g * c);
+    }
+  } else if (kind == LMC_PHASE_TABLE_THETA) {
+    theta = table_sample(v, S, uth);
+  } else if (kind == LMC_PHASE_TABLE_COS) {
+    theta = M<R>::acos(table_sample(v, S, uth));
+  } else {
+    theta = M<R>::acos(R(2) * uth - R(1));
+  }
+  R st, ct;
+  M<R>::sincos(theta, &st, &ct);
+  cx<R> s1, s2;
+  s_lookup(v.stab, S, theta, s1, s2);
+  // azimuth by rejection against F(phi) = a cos^2 + b sin^2 + c sin cos (phase.cpp:27-49)
+  const R s2sq = cnorm(s2), s1sq = cnorm(s1);
+  const R e1sq = cnorm(ph.Em), e2sq = cnorm(ph.En);
+  const R e12 = ph.Em.r * ph.En.r + ph.Em.i * ph.En.i;
+  const R a = s2sq * e1sq + s1sq * e2sq;
+  const R b = s1sq * e1sq + s2sq * e2sq;
+  const R c = R(2) * (s2sq - s1sq) * e12;
+  const R diff = a - b;
+  const R Fmax = R(0.5) * (a + b) + R(0.5) * M<R>::sqrt(diff * diff + c * c);
+  R cph = R(1), sph = R(0), F = R(0);
+  int trials = 0;
+  for (; trials < 256;) {
+    R pa, aa, pb, ab;
+    if constexpr (kXo) {
+      pa = static_cast<R>(ph.xo.s.uniform());
+      aa = static_cast<R>(ph.xo.s.uniform());
+      pb = aa;
+      ab = aa;
+    } else {
+      const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events),
+                                    1u + static_cast<uint32_t>(trials >> 1));
+      pa = u01<R>(blk.x);
+      aa = u01<R>(blk.y);
+      pb = u01<R>(blk.z);
+      ab = u01<R>(blk.w);
+    }
+    R sp_, cp_;
+    M<R>::sincos_rev(pa, &sp_, &cp_);
+    R Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
+    ++trials;
+    if (aa * Fmax <= (Ft > R(0) ? Ft : R(0))) {
+      cph = cp_;
+      sph = sp_;
+      F = Ft;
+      break;
+    }
+    if constexpr (kXo) continue;
+    // second trial of the Philox pair
+    M<R>::sincos_rev(pb, &sp_, &cp_);
+    Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
+    ++trials;
+    if (ab * Fmax <= (Ft > R(0) ? Ft : R(0))) {
+      cph = cp_;
+      sph = sp_;
+      F = Ft;
+      break;
+    }
+  }
+  // frame update P <- A(theta, phi) P (simulation.cpp:438-453)
+  const Frame<R> Po = ph.P;
+  {
+    const R ctc = ct * cph, cts = ct * sph, stc = st * cph, sts = st * sph;
+    ph.P.m = {ctc * Po.m.x + cts * Po.n.x - st * Po.s.x, ctc * Po.m.y + cts * Po.n.y - st * Po.s.y,
+              ctc * Po.m.z + cts * Po.n.z - st * Po.s.z};
+    ph.P.n = {-sph * Po.m.x + cph * Po.n.x, -sph * Po.m.y + cph * Po.n.y, -sph * Po.m.z + cph * Po.n.z};
+    ph.P.s = {stc * Po.m.x + sts * Po.n.x + ct * Po.s.x, stc * Po.m.y + sts * Po.n.y + ct * Po.s.y,
+              stc * Po.m.z + sts * Po.n.z + ct * Po.s.z};
+  }
+  // Jones update E <- S R(phi) E / sqrt(F) (simulation.cpp:456-492); F is the
+  // accepted trial's value of the same quadratic form.
+  {
+    const R inv = M<R>::rsqrt(F);
+    const cx<R> um = cadd(cscale(ph.Em, cph), cscale(ph.En, sph));
+    const cx<R> un = cadd(cscale(ph.Em, -sph), cscale(ph.En, cph));
+    ph.Em = cscale(cmul(s2, um), inv);
+    ph.En = cscale(cmul(s1, un), inv);
+  }
+  if constexpr (kTrack) {  // CBS bookkeeping (simulation.cpp:495-567)
+    const int evt = ph.events + 1;
+    if (evt == 1) {
+      ph.r1 = ph.pos;
+      ph.m1 = ph.P.m;
+      ph.s1 = ph.P.s;
+      ph.sp = (sp << 16) | sp;
+    }
+    ph.rn = ph.pos;
+    ph.sp = (ph.sp & 0xFFFF0000) | sp;
+    ph.mq = Po.m;
+    ph.sq = Po.s;
+    if (evt >= 3) {
+      // commit the pending J_{n-1} = diag(b2, b1) [[bc, bs], [-bs, bc]]:  T <- J T, Frobenius-normalised
+      const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
+      const cx<R> r1 = cadd(cscale(ph.T[1], ph.bc), cscale(ph.T[3], ph.bs));
+      const cx<R> r2 = cadd(cscale(ph.T[0], -ph.bs), cscale(ph.T[2], ph.bc));
+      const cx<R> r3 = cadd(cscale(ph.T[1], -ph.bs), cscale(ph.T[3], ph.bc));
+      const cx<R> t0 = cmul(ph.b2, r0), t1 = cmul(ph.b2, r1), t2 = cmul(ph.b1, r2), t3 = cmul(ph.b1, r3);
+      const R fr2 = cnorm(t0) + cnorm(t1) + cnorm(t2) + cnorm(t3);
+      const R inv = fr2 > R(0) ? M<R>::rsqrt(fr2) : R(1);
+      ph.T[0] = cscale(t0, inv);
+      ph.T[1] = cscale(t1, inv);
+      ph.T[2] = cscale(t2, inv);
+      ph.T[3] = cscale(t3, inv);
+    }
+    ph.b2 = s2;
+    ph.b1 = s1;
+    ph.bc = cph;
+    ph.bs = sph;
+  }
+  if constexpr (kAdj) {
+#pragma unroll
+    for (int l = 0; l < kMaxGradLayers; ++l)
+      if (l == ph.layer) ph.aN[l] += R(1);
+  }
+  // weight, roulette, cap (simulation.cpp:578-613)
+  ph.events++;
+  if (Ld.mu_t > R(0)) ph.w *= Ld.albedo;
+  if (ph.w < static_cast<R>(1e-4)) {
+    R ur;
+    if constexpr (kXo)
+      ur = static_cast<R>(ph.xo.s.uniform());
+    else
+      ur = ph.u_rr;
+    if (ur < static_cast<R>(0.1))
+      ph.w /= static_cast<R>(0.1);
+    else
+      ph.alive = false;
+  }
+  if (ph.events > p.max_events) ph.alive = false;
+  return trials;
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj>
+__global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  unsigned char* q = smem;
+  auto carve = [&](size_t bytes) {
+    unsigned char* r = q;
+    q += (bytes + 15) & ~static_cast<size_t>(15);
+    return r;
+  };
+  LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(carve(sizeof(LayerDev<R>) * p.n_layers));
+  SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(carve(sizeof(SpeciesDev<R>) * p.n_species));
+  SensorDev* sSen = reinterpret_cast<SensorDev*>(carve(sizeof(SensorDev) * p.n_sensors));
+  PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
+  unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
+  const int tid = threadIdx.x, nt = blockDim.x;
+  auto copy = [&](void* dst, const void* src, size_t bytes) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (int i = tid; i < static_cast<int>(bytes / 4); i += nt) d[i] = s[i];
+  };
+  copy(sL, p.layers, sizeof(LayerDev<R>) * p.n_layers);
+  copy(sS, p.species, sizeof(SpeciesDev<R>) * p.n_species);
+  copy(sSen, p.sensors, sizeof(SensorDev) * p.n_sensors);
+  copy(sPl, p.planes, sizeof(PlaneDev) * p.n_planes);
+  for (int i = tid; i < kMaxSensors + 2; i += nt) sHits[i] = 0u;
+  SceneView<R> v;
+  v.L = sL;
+  v.S = sS;
+  v.sens = sSen;
+  v.planes = sPl;
+  v.lgrid = nullptr;
+  v.lhits = sHits;
+  if constexpr (kLdsTables) {
+    R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));
+    uint16_t* tg = reinterpret_cast<uint16_t*>(carve(sizeof(uint16_t) * p.guide_total + 4));
+    STab<R>* ts = reinterpret_cast<STab<R>*>(carve(sizeof(STab<R>) * p.stab_total));
+    R* tsel = reinterpret_cast<R*>(carve(sizeof(R) * p.sel_total));
+    copy(tc, p.cdf, sizeof(R) * p.cdf_total);
+    for (int i = tid; i < p.guide_total; i += nt) tg[i] = p.guide[i];
+    copy(ts, p.stab, sizeof(STab<R>) * p.stab_total);
+    copy(tsel, p.sel, sizeof(R) * p.sel_total);
+    v.cdf = tc;
+    v.guide = tg;
+    v.stab = ts;
+    v.sel = tsel;
+  } else {
+    v.cdf = p.cdf;
+    v.guide = p.guide;
+    v.stab = p.stab;
+    v.sel = p.sel;
+  }
+  __syncthreads();
+
+  // persistent photon loop with wave-level refill from a claimed index pool
+  Photon<R, kXo> ph;
+  ph.alive = false;
+  uint64_t w_ph = 0, w_ev = 0, w_tr = 0;  // wave-uniform counters
+  uint64_t pool_next = 0, pool_end = 0;
+  bool exhausted = false;
+  const int lane = __lane_id();
+  const uint64_t below_mask = (1ull << lane) - 1ull;
+  for (;;) {
+    const uint64_t dead = __ballot(!ph.alive);
+    if (dead != 0 && !exhausted) {
+      if (pool_next >= pool_end) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(p.counter, static_cast<unsigned long long>(p.claim));
+        base = uniform_u64(__shfl(base, 0));
+        if (base >= p.n_photons) {
+          exhausted = true;
+          pool_next = pool_end = p.n_photons;
+        } else {
+          pool_next = base;
+          pool_end = base + p.claim < p.n_photons ? base + p.claim : p.n_photons;
+        }
+      }
+      const uint64_t my = pool_next + static_cast<uint64_t>(__popcll(dead & below_mask));
+      const bool take = !ph.alive && my < pool_end;
+      if (take) launch<R, kXo>(p, ph, static_cast<uint32_t>(p.offset + my));
+      const uint64_t nd = static_cast<uint64_t>(__popcll(dead));
+      const uint64_t avail = pool_end - pool_next;
+      const uint64_t used = nd < avail ? nd : avail;
+      pool_next += used;
+      w_ph += used;
+    }
+    const uint64_t alive = __ballot(ph.alive);
+    if (alive == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    int32_t ev0 = ph.events;
+    if (ph.alive) {
+      const int tr = transport_event<R, kXo, kTrack, kAdj>(p, v, ph);
+      w_tr += tr;  // per-lane trial counts summed below
+    }
+    w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));
+  }
+  // reduce the per-lane trial sums across the wave
+  for (int o = 32; o > 0; o >>= 1) w_tr += __shfl_xor(w_tr, o);
+  if (lane == 0) {
+    atomicAdd(&p.stats[0], static_cast<unsigned long long>(w_ph));
+    atomicAdd(&p.stats[2], static_cast<unsigned long long>(w_tr));
+  }
+  __syncthreads();
+  if (tid == 0) atomicAdd(&p.stats[3], static_cast<unsigned long long>(sHits[kMaxSensors]));
+  if (lane == 0) atomicAdd(&p.stats[1], static_cast<unsigned long long>(w_ev));
+}
+
+// Sensor kernel: one thread per queued crossing.  For every sensor that
+// accepted it, the process_hit of its kind: FarFieldCBSSensor with the
+// time-reversed partner's Jones vector and the CBS path phase
+// exp(i k0 (s_out + s_in).(r_n - r_1)) (detector.cpp:1231-1310),
+// PlanarFluenceSensor (:780-841), PlanarFieldSensor with the FP64
+// propagation phase exp(i k0 optical_path) (:546-568), and the far-field
+// fluence and reflectance sensors.  Small grids accumulate in a per-workgroup
+// LDS copy (ds_add_f64), the rest straight to HBM (global_atomic_add_f64).
+template <class R, bool kAdj>
+__global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* sgrid = reinterpret_cast<double*>(smem);
+  unsigned* shits = reinterpret_cast<unsigned*>(smem + ((sizeof(double) * p.lds_grid + 15) & ~static_cast<size_t>(15)));
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < p.lds_grid; i += nt) sgrid[i] = 0.0;
+  for (int i = tid; i < kMaxSensors + 1; i += nt) shits[i] = 0u;
+  __syncthreads();
+  const int64_t n_rec = static_cast<int64_t>(p.q_count[0]) < p.q_cap ? static_cast<int64_t>(p.q_count[0]) : p.q_cap;
+  const int64_t c = p.q_cap;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * nt + tid; i < n_rec; i += static_cast<int64_t>(gridDim.x) * nt) {
+    const R* f = p.qf + i;
+    const int32_t* iq = p.qi + i;
+    const int pl = iq[I_PLANE * c];
+    const int mask = iq[I_MASK * c];
+    const int events = iq[I_EVENTS * c];
+    const R hx = f[F_HX * c], hy = f[F_HY * c];
+    const R arr = f[F_ARR * c];
+    const R w = f[F_W * c];
+    const cx<R> Em{f[(F_EM + 0) * c], f[(F_EM + 1) * c]}, En{f[(F_EM + 2) * c], f[(F_EM + 3) * c]};
+    Frame<R> P;
+    P.m = {f[(F_P + 0) * c], f[(F_P + 1) * c], f[(F_P + 2) * c]};
+    P.n = {f[(F_P + 3) * c], f[(F_P + 4) * c], f[(F_P + 5) * c]};
+    P.s = {f[(F_P + 6) * c], f[(F_P + 7) * c], f[(F_P + 8) * c]};
+    R score[3 * kMaxGradLayers];
+    if constexpr (kAdj) {
+#pragma unroll
+      for (int qq = 0; qq < 3 * kMaxGradLayers; ++qq) score[qq] = f[(F_SCORE + qq) * c];
+    }
+    const R ws = M<R>::sqrt(w);
+    const cx<R> fm = cscale(Em, ws), fn = cscale(En, ws);
+    // forward field on the lab axes (project_to_lab, detector.cpp:1048-1053)
+    const cx<R> Efx = cadd(cscale(fm, P.m.x), cscale(fn, P.n.x));
+    const cx<R> Efy = cadd(cscale(fm, P.m.y), cscale(fn, P.n.y));
+    const int first = p.planes[pl].first, cnt = p.planes[pl].count;
+    for (int k = 0; k < cnt; ++k) {
+      if (!((mask >> k) & 1)) continue;
+      const int si = first + k;
+      const SensorDev& sd = p.sensors[si];
+      bool counted = false;
+      if (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_PLANAR_FLUENCE) {
+        // time bin (detector.cpp:1182-1190, :787-797)
+        int t_idx = sd.kind == LMC_SENSOR_FARFIELD_CBS ? 0 : -1;
+        if (sd.g[5] > 0) {
+          const double a = static_cast<double>(arr);
+          if (a < 0 || a >= sd.g[2]) continue;
+          t_idx = static_cast<int>(a / sd.g[5]) + 1;
+          if (t_idx >= sd.n2) continue;
+        }
+        const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
+        const int64_t plane = tplane * sd.n2;
+        if (sd.kind == LMC_SENSOR_FARFIELD_CBS) {
+          const int it = static_cast<int>(exit_theta(P.s) / static_cast<R>(sd.g[3]));
+          const int jp = static_cast<int>(exit_phi(P.s) / static_cast<R>(sd.g[4]));
+          if (it < 0 || it >= sd.n0 || jp < 0 || jp >= sd.n1) continue;
+          const int64_t bin = static_cast<int64_t>(it) * sd.n1 + jp;
+          cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
+          if (events >= 2) {
+            const int spk = iq[I_SP * c];
+            const Frame<R> P1 = frame_of(V3<R>{f[(F_M1 + 0) * c], f[(F_M1 + 1) * c], f[(F_M1 + 2) * c]},
+                                         V3<R>{f[(F_S1 + 0) * c], f[(F_S1 + 1) * c], f[(F_S1 + 2) * c]});
+            const Frame<R> Pq = frame_of(V3<R>{f[(F_MQ + 0) * c], f[(F_MQ + 1) * c], f[(F_MQ + 2) * c]},
+                                         V3<R>{f[(F_SQ + 0) * c], f[(F_SQ + 1) * c], f[(F_SQ + 2) * c]});
+            cx<R> T[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) T[kk] = {f[(F_T + 2 * kk) * c], f[(F_T + 2 * kk + 1) * c]};
+            const V3<R> d{f[(F_D + 0) * c], f[(F_D + 1) * c], f[(F_D + 2) * c]};
+            cx<R> rm, rn;
+            reverse_field(p, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
+            const R arg =
+                p.k0 * ((P.s.x + p.ldir[0]) * d.x + (P.s.y + p.ldir[1]) * d.y + (P.s.z + p.ldir[2]) * d.z);
+            R sa, ca;
+            M<R>::sincos(arg, &sa, &ca);
+            const cx<R> pp{ca * ws, sa * ws};
+            rm = cmul(rm, pp);
+            rn = cmul(rn, pp);
+            Erx = cadd(cscale(rm, P.m.x), cscale(rn, P.n.x));
+            Ery = cadd(cscale(rm, P.m.y), cscale(rn, P.n.y));
+            atomicAdd(&shits[kMaxSensors], 1u);
+          }
+          cbs_deposit<R, kAdj>(p, sd, sgrid, bin, t_idx, Efx, Efy, Erx, Ery, score);
+          counted = true;
+        } else {
+          const int xi = static_cast<int>((static_cast<double>(hx) + 0.5 * sd.g[0]) / sd.g[3]);
+          const int yi = static_cast<int>((static_cast<double>(hy) + 0.5 * sd.g[1]) / sd.g[4]);
+          if (xi < 0 || xi >= sd.n0 || yi < 0 || yi >= sd.n1) continue;
+          const cx<R> Ex = cscale(Efx, R(-1));  // lab x negated (detector.cpp:812)
+          const int64_t bin = static_cast<int64_t>(xi) * sd.n1 + yi;
+#pragma unroll 1
+          for (int q = 0; q < 4; ++q) {
+            const double dv = static_cast<double>(stokes(q, Ex, Efy));
+            if (t_idx >= 1) deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, dv, score);
+            deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, dv, score);
+          }
+          counted = true;
+        }
+      } else if (sd.kind == LMC_SENSOR_PLANAR_FIELD) {
+        const int xi = static_cast<int>((static_cast<double>(hx) + 0.5 * sd.g[0]) / sd.g[2]);
+        const int yi = static_cast<int>((static_cast<double>(hy) + 0.5 * sd.g[1]) / sd.g[3]);
+        if (xi < 0 || xi >= sd.n0 || yi < 0 || yi >= sd.n1) continue;
+        double sp, cp;
+        ::sincos(static_cast<double>(p.k0) * p.qd[i], &sp, &cp);
+        const double wsd = ::sqrt(static_cast<double>(w));
+        const double exr = static_cast<double>(Em.r * P.m.x + En.r * P.n.x);
+        const double exi = static_cast<double>(Em.i * P.m.x + En.i * P.n.x);
+        const double eyr = static_cast<double>(Em.r * P.m.y + En.r * P.n.y);
+        const double eyi = static_cast<double>(Em.i * P.m.y + En.i * P.n.y);
+        const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
+        const int64_t bin = static_cast<int64_t>(xi) * sd.n1 + yi;
+        deposit<R, kAdj>(p, sd, sgrid, 2 * bin, (exr * cp - exi * sp) * wsd, score);
+        deposit<R, kAdj>(p, sd, sgrid, 2 * bin + 1, (exr * sp + exi * cp) * wsd, score);
+        deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin), (eyr * cp - eyi * sp) * wsd, score);
+        deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin) + 1, (eyr * sp + eyi * cp) * wsd, score);
+        counted = true;
+      } else if (sd.kind == LMC_SENSOR_FARFIELD_FLUENCE) {
+        const int it = static_cast<int>(exit_theta(P.s) / static_cast<R>(sd.g[2]));
+        const int jp = static_cast<int>(exit_phi(P.s) / static_cast<R>(sd.g[3]));
+        if (it < 0 || it >= sd.n0 || jp < 0 || jp >= sd.n1) continue;
+        const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
+        const int64_t bin = static_cast<int64_t>(it) * sd.n1 + jp;
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q)
+          deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, static_cast<double>(stokes(q, Efx, Efy)), score);
+        counted = true;
+      } else if (sd.kind == LMC_SENSOR_REFLECTANCE) {
+        deposit<R, kAdj>(p, sd, sgrid, 0, static_cast<double>(w), score);
+        counted = true;
+      }
+      if (counted) atomicAdd(&shits[si], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < p.n_sensors; i += nt)
+    if (shits[i]) atomicAdd(&p.hits[i], static_cast<unsigned long long>(shits[i]));
+  if (tid == 0 && shits[kMaxSensors]) atomicAdd(&p.stats[4], static_cast<unsigned long long>(shits[kMaxSensors]));
+  for (int si = 0; si < p.n_sensors; ++si) {
+    const SensorDev& sd = p.sensors[si];
+    if (!sd.lds) continue;
+    const int64_t len = (sd.kind == LMC_SENSOR_FARFIELD_CBS)
+                            ? 8ll * sd.n0 * sd.n1 * sd.n2
+                            : (sd.kind == LMC_SENSOR_REFLECTANCE ? 1ll : 4ll * sd.n0 * sd.n1);
+    for (int64_t i = tid; i < len; i += nt) {
+      const double val = sgrid[sd.lds_off + i];
+      if (val != 0.0) unsafeAtomicAdd(p.out + sd.out_off + i, val);
+    }
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_philox_probe(int n, const uint4* ctr, uint32_t k0, uint32_t k1, uint4* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
+}
+
+}  // namespace lmc
