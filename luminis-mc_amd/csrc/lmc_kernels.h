@@ -63,7 +63,18 @@ enum RecF {
 };
 enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, kRecI = 4 };
 
+// Launch-only constants (laser), read from LDS by the refill path so they do
+// not occupy scalar registers across the event loop.
+template <class R> struct LaunchConst {
+  double lpos_x, lpos_y, sigma, pulse, rep, t_off;
+  R lpos_z;
+  R ldir[3], lm[3], ln[3];
+  R pol[4];
+  int32_t source, temporal, launch_layer, pad;
+};
+
 template <class R> struct SceneView {
+  const LaunchConst<R>* lc;
   const LayerDev<R>* L;
   const SpeciesDev<R>* S;
   const R* cdf;
@@ -113,7 +124,13 @@ template <class R> __device__ __forceinline__ V3<R> cross3(V3<R> a, V3<R> b) {
 
 __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t key1, uint32_t idx, uint32_t idx_hi,
                                              uint32_t ev, uint32_t sub) {
-  return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), key0, key1);
+  // The key is uniform, but keeping it in vector registers stops the
+  // compiler from hoisting all 20 round keys of the schedule into scalar
+  // registers for the whole event loop (they would spill).
+  uint32_t k0v, k1v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(k0v) : "s"(key0));
+  asm volatile("v_mov_b32 %0, %1" : "=v"(k1v) : "s"(key1));
+  return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), k0v, k1v);
 }
 
 // Inverse-CDF lookup: SamplingTable::Sample (src/table.cpp:50-66).  The
@@ -373,7 +390,15 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
     if (sd.absorb) killed = true;
   }
   if (mask == 0) return killed;
-  const unsigned long long slot = atomicAdd(&p.q_count[0], 1ull);
+  // one queue reservation per wave: the recording lanes get consecutive slots,
+  // so their field-major stores coalesce
+  const uint64_t act = __ballot(1);
+  const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
+  const int lane = __lane_id();
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(&p.q_count[0], static_cast<unsigned long long>(__popcll(act)));
+  base = __shfl(base, leader);
+  const unsigned long long slot = base + static_cast<unsigned long long>(__popcll(act & ((1ull << lane) - 1ull)));
   if (static_cast<int64_t>(slot) >= p.q_cap) {
     atomicAdd(&p.q_count[1], 1ull);
     return killed;
@@ -438,41 +463,42 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
 // Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
 // The Philox stream draws emission uniforms from event slot 0xFFFFFFFF.
 template <class R, bool kXo>
-__device__ __forceinline__ void launch(const KParams<R>& p, Photon<R, kXo>& ph, uint32_t idx) {
+__device__ __forceinline__ void launch(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, uint32_t idx) {
+  const LaunchConst<R>& L = *v.lc;
   ph.idx = idx;
-  R x = static_cast<R>(p.lpos_x), y = static_cast<R>(p.lpos_y);
-  R t0 = static_cast<R>(p.t_off * 1e-3);
-  const R pulse = static_cast<R>(p.pulse * 1e-3);
+  R x = static_cast<R>(L.lpos_x), y = static_cast<R>(L.lpos_y);
+  R t0 = static_cast<R>(L.t_off * 1e-3);
+  const R pulse = static_cast<R>(L.pulse * 1e-3);
   const R fwhm_to_sigma = static_cast<R>(1.0 / (2.0 * 1.1774100225154747));  // 1/(2 sqrt(2 ln 2))
   if constexpr (kXo) {
     ph.xo.init((static_cast<uint64_t>(p.key1) << 32) | p.key0, (static_cast<uint64_t>(p.idx_hi) << 32) | idx);
-    double xd = p.lpos_x, yd = p.lpos_y, td = p.t_off * 1e-3;
+    double xd = L.lpos_x, yd = L.lpos_y, td = L.t_off * 1e-3;
     const double kTwoPi = 2.0 * 3.14159265358979323846;
-    if (p.source == LMC_SOURCE_UNIFORM) {
+    if (L.source == LMC_SOURCE_UNIFORM) {
       const double th = ph.xo.s.uniform() * kTwoPi;
-      const double r = ::sqrt(ph.xo.s.uniform()) * p.sigma;
+      const double r = ::sqrt(ph.xo.s.uniform()) * L.sigma;
       xd += r * ::cos(th);
       yd += r * ::sin(th);
-    } else if (p.source == LMC_SOURCE_GAUSSIAN) {
-      xd = ph.xo.s.normal(xd, p.sigma);
-      yd = ph.xo.s.normal(yd, p.sigma);
+    } else if (L.source == LMC_SOURCE_GAUSSIAN) {
+      xd = ph.xo.s.normal(xd, L.sigma);
+      yd = ph.xo.s.normal(yd, L.sigma);
     }
-    switch (p.temporal) {
-      case LMC_TIME_GAUSSIAN: td = ph.xo.s.normal(td, p.pulse * 1e-3 / (2.0 * 1.1774100225154747)); break;
-      case LMC_TIME_TOPHAT: td = td + (ph.xo.s.uniform() - 0.5) * p.pulse * 1e-3; break;
-      case LMC_TIME_EXPONENTIAL: td = td - ::log(ph.xo.s.uniform()) * p.pulse * 1e-3; break;
-      case LMC_TIME_PULSE_TRAIN: if (p.rep > 0.0) td = td + ph.xo.s.uniform() * (1e9 / p.rep); break;
-      case LMC_TIME_CW: if (p.pulse > 0.0) td = td + ph.xo.s.uniform() * p.pulse * 1e-3; break;
+    switch (L.temporal) {
+      case LMC_TIME_GAUSSIAN: td = ph.xo.s.normal(td, L.pulse * 1e-3 / (2.0 * 1.1774100225154747)); break;
+      case LMC_TIME_TOPHAT: td = td + (ph.xo.s.uniform() - 0.5) * L.pulse * 1e-3; break;
+      case LMC_TIME_EXPONENTIAL: td = td - ::log(ph.xo.s.uniform()) * L.pulse * 1e-3; break;
+      case LMC_TIME_PULSE_TRAIN: if (L.rep > 0.0) td = td + ph.xo.s.uniform() * (1e9 / L.rep); break;
+      case LMC_TIME_CW: if (L.pulse > 0.0) td = td + ph.xo.s.uniform() * L.pulse * 1e-3; break;
       default: break;
     }
     x = static_cast<R>(xd);
     y = static_cast<R>(yd);
     t0 = static_cast<R>(td);
   } else {
-    if (p.source != LMC_SOURCE_POINT) {
+    if (L.source != LMC_SOURCE_POINT) {
       const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 0u);
-      const R sg = static_cast<R>(p.sigma);
-      if (p.source == LMC_SOURCE_UNIFORM) {
+      const R sg = static_cast<R>(L.sigma);
+      if (L.source == LMC_SOURCE_UNIFORM) {
         R sn, cs;
         M<R>::sincos_rev(u01<R>(b.x), &sn, &cs);
         const R r = M<R>::sqrt(u01<R>(b.y)) * sg;
@@ -486,10 +512,10 @@ __device__ __forceinline__ void launch(const KParams<R>& p, Photon<R, kXo>& ph, 
         y += M<R>::sqrt(R(-2) * M<R>::log(u01<R>(b.z))) * cs * sg;
       }
     }
-    if (p.temporal != LMC_TIME_DELTA) {
+    if (L.temporal != LMC_TIME_DELTA) {
       const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 1u);
       const R ua = u01<R>(b.x);
-      switch (p.temporal) {
+      switch (L.temporal) {
         case LMC_TIME_GAUSSIAN: {
           R sn, cs;
           M<R>::sincos_rev(u01<R>(b.y), &sn, &cs);
@@ -498,24 +524,24 @@ __device__ __forceinline__ void launch(const KParams<R>& p, Photon<R, kXo>& ph, 
         }
         case LMC_TIME_TOPHAT: t0 += (ua - R(0.5)) * pulse; break;
         case LMC_TIME_EXPONENTIAL: t0 -= M<R>::log(ua) * pulse; break;
-        case LMC_TIME_PULSE_TRAIN: if (p.rep > 0.0) t0 += ua * static_cast<R>(1e9 / p.rep); break;
-        case LMC_TIME_CW: if (p.pulse > 0.0) t0 += ua * pulse; break;
+        case LMC_TIME_PULSE_TRAIN: if (L.rep > 0.0) t0 += ua * static_cast<R>(1e9 / L.rep); break;
+        case LMC_TIME_CW: if (L.pulse > 0.0) t0 += ua * pulse; break;
         default: break;
       }
     }
   }
-  ph.pos = {x, y, p.lpos_z};
-  ph.P.m = {p.lm[0], p.lm[1], p.lm[2]};
-  ph.P.n = {p.ln[0], p.ln[1], p.ln[2]};
-  ph.P.s = {p.ldir[0], p.ldir[1], p.ldir[2]};
-  ph.Em = {p.pol[0], p.pol[1]};
-  ph.En = {p.pol[2], p.pol[3]};
+  ph.pos = {x, y, L.lpos_z};
+  ph.P.m = {L.lm[0], L.lm[1], L.lm[2]};
+  ph.P.n = {L.ln[0], L.ln[1], L.ln[2]};
+  ph.P.s = {L.ldir[0], L.ldir[1], L.ldir[2]};
+  ph.Em = {L.pol[0], L.pol[1]};
+  ph.En = {L.pol[2], L.pol[3]};
   ph.w = R(1);
   ph.gpl = R(0);
   ph.t0 = t0;
   ph.opl = 0.0;
   ph.events = 0;
-  ph.layer = p.launch_layer;
+  ph.layer = L.launch_layer;
   ph.alive = true;
   ph.m1 = ph.P.m;
   ph.s1 = ph.P.s;
@@ -819,6 +845,16 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
   SensorDev* sSen = reinterpret_cast<SensorDev*>(carve(sizeof(SensorDev) * p.n_sensors));
   PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
   unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
+  LaunchConst<R>* sLC = reinterpret_cast<LaunchConst<R>*>(carve(sizeof(LaunchConst<R>)));
+  if (threadIdx.x == 0) {
+    LaunchConst<R> c;
+    c.lpos_x = p.lpos_x; c.lpos_y = p.lpos_y; c.sigma = p.sigma; c.pulse = p.pulse; c.rep = p.rep; c.t_off = p.t_off;
+    c.lpos_z = p.lpos_z;
+    for (int j = 0; j < 3; ++j) { c.ldir[j] = p.ldir[j]; c.lm[j] = p.lm[j]; c.ln[j] = p.ln[j]; }
+    for (int j = 0; j < 4; ++j) c.pol[j] = p.pol[j];
+    c.source = p.source; c.temporal = p.temporal; c.launch_layer = p.launch_layer; c.pad = 0;
+    *sLC = c;
+  }
   const int tid = threadIdx.x, nt = blockDim.x;
   auto copy = [&](void* dst, const void* src, size_t bytes) {
     const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
@@ -837,6 +873,7 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
   v.planes = sPl;
   v.lgrid = nullptr;
   v.lhits = sHits;
+  v.lc = sLC;
   if constexpr (kLdsTables) {
     R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));
     uint16_t* tg = reinterpret_cast<uint16_t*>(carve(sizeof(uint16_t) * p.guide_total + 4));
@@ -883,7 +920,7 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
       }
       const uint64_t my = pool_next + static_cast<uint64_t>(__popcll(dead & below_mask));
       const bool take = !ph.alive && my < pool_end;
-      if (take) launch<R, kXo>(p, ph, static_cast<uint32_t>(p.offset + my));
+      if (take) launch<R, kXo>(p, v, ph, static_cast<uint32_t>(p.offset + my));
       const uint64_t nd = static_cast<uint64_t>(__popcll(dead));
       const uint64_t avail = pool_end - pool_next;
       const uint64_t used = nd < avail ? nd : avail;

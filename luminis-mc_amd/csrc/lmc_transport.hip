@@ -384,6 +384,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st
   add(sizeof(SensorDev) * p.n_sensors);
   add(sizeof(PlaneDev) * p.n_planes);
   add(sizeof(unsigned) * (kMaxSensors + 2));
+  add(sizeof(LaunchConst<R>));
   const size_t tab_lds = ((sizeof(R) * T.cdf_total + 15) & ~15ull) +
                          ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
                          ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
