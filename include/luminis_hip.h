@@ -191,7 +191,7 @@ typedef struct lmc_stats {
   uint64_t cbs_detections; /* photons deposited with a reverse path (events>=2) */
   double kernel_ms;        /* device time of all kernels of the run           */
   double transport_ms;     /* device time of the transport kernel launches    */
-  double records_ms;       /* device time of the CBS reverse-path kernel      */
+  double sensor_ms;        /* device time of the sensor (crossing) kernel     */
   uint64_t transport_launches;
 } lmc_stats;
 

@@ -475,7 +475,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st
     st->cbs_detections = c[5];
     st->kernel_ms = ms_t + ms_r;
     st->transport_ms = ms_t;
-    st->records_ms = ms_r;
+    st->sensor_ms = ms_r;
     st->transport_launches = launches;
   }
   return LMC_OK;

@@ -105,7 +105,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("photons", c_uint64), ("events", c_uint64), ("phi_trials", c_uint64),
         ("crossings", c_uint64), ("cbs_detections", c_uint64),
-        ("kernel_ms", c_double), ("transport_ms", c_double), ("records_ms", c_double),
+        ("kernel_ms", c_double), ("transport_ms", c_double), ("sensor_ms", c_double),
         ("transport_launches", c_uint64),
     ]
 
