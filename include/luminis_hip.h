@@ -235,16 +235,23 @@ int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits);
 int lmc_scene_zero(lmc_scene* s);
 
 /* ---- batched persistent launch over many independent scenes ----------- */
+/* No counterpart in the reference (which runs one SimConfig per call): up to
+ * 4096 scenes, each validated as by lmc_scene_create, share one persistent
+ * transport launch.  Scene i's photon j draws the same Philox (or xoshiro)
+ * stream as photon j of lmc_scene_run on that scene alone with the same seed,
+ * so a batched scene reproduces its single-scene run (up to the summation
+ * order of the FP64 accumulators). */
 typedef struct lmc_batch lmc_batch;
-/* All scenes must hold exactly one sensor of kind LMC_SENSOR_FARFIELD_CBS
- * (plus optionally one LMC_SENSOR_REFLECTANCE). */
 int lmc_batch_create(int32_t n_scenes, const lmc_scene_desc* scenes, lmc_batch** out);
 void lmc_batch_destroy(lmc_batch* b);
-/* Run budgets[i] photons of scene i (photon indices from 0) in ONE launch;
- * if use_graph != 0 the zero+transport sequence is captured once into a
- * hipGraph and replayed.  Adds into the per-scene device accumulators. */
+/* Run budgets[i] photons of scene i (photon indices photon_offset + [0,
+ * budgets[i]), below 2^32; run->n_photons is ignored) and add into the
+ * per-scene device accumulators.  use_graph != 0: the launch sequence is
+ * captured once into a hipGraph and replayed while budgets and run repeat. */
 int lmc_batch_run(lmc_batch* b, const uint64_t* budgets, const lmc_run_desc* run,
                   int32_t use_graph, lmc_stats* stats);
+/* Add scene `scene`'s accumulators / hit counters into host buffers, as
+ * lmc_scene_read. */
 int lmc_batch_read(lmc_batch* b, int32_t scene, double* out, int64_t n, uint64_t* hits);
 int lmc_batch_zero(lmc_batch* b);
 

@@ -252,11 +252,18 @@ __device__ __forceinline__ void stab_get(const STab<double>& t, cx<double>& s1, 
   s2 = {t.b.x, t.b.y};
 }
 
+// One layer with what the event loop needs of its scene, so that layers of
+// several scenes can share one array: neighbour flags (bit 0: a layer above in
+// +z, bit 1: a layer below), the scene's top face (+inf when semi-infinite)
+// and the scene's sensor planes [pl_first, pl_first + pl_count).
 template <class R> struct LayerDev {
   R z_min, z_max;  // z_max = +inf for a semi-infinite last layer
   R mfp, mu_t, albedo, n;
   R mu_s, mu_a;
-  int32_t sp_first, sp_count, sel_off, pad;
+  int32_t sp_first, sp_count, sel_off, nbr;
+  int32_t pl_first, pl_count;
+  R z_top;
+  int32_t pad;
 };
 
 struct SensorDev {
@@ -268,11 +275,15 @@ struct SensorDev {
   double g[8];
   int64_t out_off;
   int32_t lds_off, pad2;
+  // conservative grid bounds for the transport kernel's queueing test: exit
+  // cosine -s_z >= q_cmin, hit point in [q_x0, q_x1] x [q_y0, q_y1], arrival
+  // in [q_t0, q_t1] (slightly wider than the grid; the sensor kernel bins exactly)
+  float q_cmin, q_x0, q_x1, q_y0, q_y1, q_t0, q_t1, pad3;
 };
 
 struct PlaneDev {
   double z;
-  int32_t first, count, has_cbs, mult;  // mult: crossing records per photon, worst case
+  int32_t first, count, has_cbs, scene;  // scene: index of its laser constants
 };
 
 }  // namespace lmc

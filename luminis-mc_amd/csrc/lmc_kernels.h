@@ -15,6 +15,17 @@
 
 namespace lmc {
 
+// Laser constants of one scene (Laser, laser.hpp:45-90), with the vacuum
+// wavenumber and the absolute index of the layer holding the launch point.
+template <class R> struct LaunchConst {
+  double lpos_x, lpos_y, sigma, pulse, rep, t_off;
+  R lpos_z;
+  R ldir[3], lm[3], ln[3];
+  R pol[4];
+  R k0;
+  int32_t source, temporal, launch_layer;
+};
+
 template <class R> struct KParams {
   const LayerDev<R>* layers;
   const SpeciesDev<R>* species;
@@ -24,6 +35,8 @@ template <class R> struct KParams {
   const R* sel;
   const SensorDev* sensors;
   const PlaneDev* planes;
+  const LaunchConst<R>* lcs;     // one per scene
+  const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
   double* out;
   unsigned long long* hits;
   unsigned long long* counter;
@@ -36,18 +49,11 @@ template <class R> struct KParams {
   int64_t q_cap;
   int64_t n_out;
   uint64_t n_photons, offset;
-  int32_t n_layers, n_species, n_sensors, n_planes;
+  uint64_t local_offset;        // batched runs: first photon index within every scene
+  int32_t n_layers, n_species, n_sensors, n_planes, n_scenes;
   int32_t cdf_total, guide_total, stab_total, sel_total;
-  int32_t lds_grid, launch_layer, source, temporal;
-  int32_t max_events, claim, top_finite;
+  int32_t lds_grid, max_events, claim;
   uint32_t key0, key1, idx_hi;
-  R z_top;
-  // laser
-  double lpos_x, lpos_y, sigma, pulse, rep, t_off;
-  R lpos_z;
-  R ldir[3], lm[3], ln[3];
-  R pol[4];
-  R k0;
 };
 
 // One queued plane crossing accepted by at least one sensor of its plane:
@@ -62,16 +68,6 @@ enum RecF {
   F_SCORE = 40, kRecF = 52
 };
 enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, kRecI = 4 };
-
-// Launch-only constants (laser), read from LDS by the refill path so they do
-// not occupy scalar registers across the event loop.
-template <class R> struct LaunchConst {
-  double lpos_x, lpos_y, sigma, pulse, rep, t_off;
-  R lpos_z;
-  R ldir[3], lm[3], ln[3];
-  R pol[4];
-  int32_t source, temporal, launch_layer, pad;
-};
 
 template <class R> struct SceneView {
   const LaunchConst<R>* lc;
@@ -205,12 +201,12 @@ template <class R> __device__ __forceinline__ R exit_phi(V3<R> s) {
 // raw interior product J_{n-1}...J_2.  Returns the normalised reverse field in
 // the exit basis (m_n, n_n).
 template <class R>
-__device__ __forceinline__ void reverse_field(const KParams<R>& p, const STab<R>* stab, const SpeciesDev<R>& spn,
+__device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const STab<R>* stab, const SpeciesDev<R>& spn,
                                               const SpeciesDev<R>& sp1, const Frame<R>& P1, const Frame<R>& Pq,
                                               const Frame<R>& Pn, const cx<R> T[4], cx<R>& Erm, cx<R>& Ern) {
-  const V3<R> s0{p.ldir[0], p.ldir[1], p.ldir[2]};
-  const V3<R> m0{p.lm[0], p.lm[1], p.lm[2]};
-  const V3<R> n0{p.ln[0], p.ln[1], p.ln[2]};
+  const V3<R> s0{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
+  const V3<R> m0{lc.lm[0], lc.lm[1], lc.lm[2]};
+  const V3<R> n0{lc.ln[0], lc.ln[1], lc.ln[2]};
   // Stage A: scatter at r_n, s0 -> -s_{n-1}
   const V3<R> s_out_a{-Pq.s.x, -Pq.s.y, -Pq.s.z};
   const R th_a = M<R>::acos(clamp1(dot(s0, s_out_a)));
@@ -219,7 +215,7 @@ __device__ __forceinline__ void reverse_field(const KParams<R>& p, const STab<R>
   const V3<R> np = safe_unit(cross_unit(s0, s_out_a), n0);
   const V3<R> mp_in = safe_unit(cross_unit(np, s0), m0);
   const V3<R> mp_out = safe_unit(cross_unit(np, s_out_a), Pq.m);
-  const cx<R> E0m{p.pol[0], p.pol[1]}, E0n{p.pol[2], p.pol[3]};
+  const cx<R> E0m{lc.pol[0], lc.pol[1]}, E0n{lc.pol[2], lc.pol[3]};
   // E = S_a * rot2(mp_in, np, m0, n0) * E0
   R r00 = dot(mp_in, m0), r01 = dot(mp_in, n0), r10 = dot(np, m0), r11 = dot(np, n0);
   cx<R> xm = cadd(cscale(E0m, r00), cscale(E0n, r01));
@@ -366,6 +362,43 @@ __device__ __forceinline__ bool accepts(const SensorDev& sd, V3<R> hp, V3<R> dir
   return true;
 }
 
+// Grid cell of one accepted crossing, with the binning of the sensors'
+// process_hit: time bin (detector.cpp:787-797, :1182-1190), exit angles
+// (:1274-1281), pixel (:546-556, :800-806).  Returns false when the crossing
+// falls outside the grid, where the reference deposits nothing and counts no
+// hit.  t_idx: time bin >= 1, 0 for a CBS sensor without a time axis, -1 for
+// the other kinds without one.
+template <class R>
+__device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R arr, V3<R> dir, int64_t& bin,
+                                           int& t_idx) {
+  const int kind = sd.kind;
+  t_idx = kind == LMC_SENSOR_FARFIELD_CBS ? 0 : -1;
+  bin = 0;
+  if ((kind == LMC_SENSOR_FARFIELD_CBS || kind == LMC_SENSOR_PLANAR_FLUENCE) && sd.g[5] > 0) {
+    const double a = static_cast<double>(arr);
+    if (a < 0 || a >= sd.g[2]) return false;
+    t_idx = static_cast<int>(a / sd.g[5]) + 1;
+    if (t_idx >= sd.n2) return false;
+  }
+  if (kind == LMC_SENSOR_FARFIELD_CBS || kind == LMC_SENSOR_FARFIELD_FLUENCE) {
+    const bool cbs = kind == LMC_SENSOR_FARFIELD_CBS;
+    const int it = static_cast<int>(exit_theta(dir) / static_cast<R>(cbs ? sd.g[3] : sd.g[2]));
+    const int jp = static_cast<int>(exit_phi(dir) / static_cast<R>(cbs ? sd.g[4] : sd.g[3]));
+    if (it < 0 || it >= sd.n0 || jp < 0 || jp >= sd.n1) return false;
+    bin = static_cast<int64_t>(it) * sd.n1 + jp;
+    return true;
+  }
+  if (kind == LMC_SENSOR_PLANAR_FLUENCE || kind == LMC_SENSOR_PLANAR_FIELD) {
+    const bool flu = kind == LMC_SENSOR_PLANAR_FLUENCE;
+    const int xi = static_cast<int>((static_cast<double>(hx) + 0.5 * sd.g[0]) / (flu ? sd.g[3] : sd.g[2]));
+    const int yi = static_cast<int>((static_cast<double>(hy) + 0.5 * sd.g[1]) / (flu ? sd.g[4] : sd.g[3]));
+    if (xi < 0 || xi >= sd.n0 || yi < 0 || yi >= sd.n1) return false;
+    bin = static_cast<int64_t>(xi) * sd.n1 + yi;
+    return true;
+  }
+  return kind == LMC_SENSOR_REFLECTANCE;
+}
+
 // A crossing of plane `pl` by the segment prev -> pos: hit point and the
 // overshoot-corrected paths (detector.cpp:194-235); the sensors' filters
 // decide acceptance and absorption here, the deposits happen in
@@ -380,14 +413,19 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
   const R corr = M<R>::sqrt(dot(ov, ov));
   atomicAdd(&v.lhits[kMaxSensors], 1u);
+  const R gpl_hit = corr > R(0) ? ph.gpl - corr : ph.gpl;
+  const R arr = ph.t0 + gpl_hit;
   bool killed = false;
   int mask = 0;
   const int first = v.planes[pl].first, cnt = v.planes[pl].count;
   for (int k = 0; k < cnt; ++k) {
     const SensorDev& sd = v.sens[first + k];
     if (!accepts(sd, hp, ph.P.s, cdir, ph.events)) continue;
-    mask |= 1 << k;
     if (sd.absorb) killed = true;
+    // only crossings that can land in a sensor's grid are queued
+    if (-ph.P.s.z >= sd.q_cmin && hp.x >= sd.q_x0 && hp.x <= sd.q_x1 && hp.y >= sd.q_y0 && hp.y <= sd.q_y1 &&
+        arr >= sd.q_t0 && arr <= sd.q_t1)
+      mask |= 1 << k;
   }
   if (mask == 0) return killed;
   // one queue reservation per wave: the recording lanes get consecutive slots,
@@ -405,12 +443,11 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   }
   const int64_t c = p.q_cap;
   R* f = p.qf + slot;
-  const R gpl_hit = corr > R(0) ? ph.gpl - corr : ph.gpl;
   double opl_hit = ph.opl;
   if (corr > R(0)) opl_hit -= static_cast<double>(corr) * static_cast<double>(v.L[ph.layer].n);
   f[F_HX * c] = hp.x;
   f[F_HY * c] = hp.y;
-  f[F_ARR * c] = ph.t0 + gpl_hit;
+  f[F_ARR * c] = arr;
   f[F_W * c] = ph.w;
   f[(F_EM + 0) * c] = ph.Em.r;
   f[(F_EM + 1) * c] = ph.Em.i;
@@ -452,7 +489,8 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
   const R zmin = z1 < z2 ? z1 : z2, zmax = z1 < z2 ? z2 : z1;
   bool killed = false;
-  for (int pl = 0; pl < p.n_planes; ++pl) {
+  const int pl0 = v.L[ph.layer].pl_first, pl1 = pl0 + v.L[ph.layer].pl_count;
+  for (int pl = pl0; pl < pl1; ++pl) {
     const R zp = static_cast<R>(v.planes[pl].z);
     if (zp >= zmin && zp <= zmax)
       killed |= plane_hit<R, kXo, kAdj>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
@@ -463,8 +501,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
 // Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
 // The Philox stream draws emission uniforms from event slot 0xFFFFFFFF.
 template <class R, bool kXo>
-__device__ __forceinline__ void launch(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, uint32_t idx) {
-  const LaunchConst<R>& L = *v.lc;
+__device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>& L, Photon<R, kXo>& ph, uint32_t idx) {
   ph.idx = idx;
   R x = static_cast<R>(L.lpos_x), y = static_cast<R>(L.lpos_y);
   R t0 = static_cast<R>(L.t_off * 1e-3);
@@ -597,12 +634,12 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     R zi = R(0);
     bool has = false;
     if (new_z > ph.pos.z) {
-      if (ph.layer + 1 < p.n_layers) {
+      if (v.L[ph.layer].nbr & 1) {
         zi = v.L[ph.layer].z_max;
         has = zi <= new_z;
       }
     } else if (new_z < ph.pos.z) {
-      if (ph.layer > 0) {
+      if (v.L[ph.layer].nbr & 2) {
         zi = v.L[ph.layer].z_min;
         has = zi >= new_z;
       }
@@ -648,7 +685,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     }
   }
   // Sample::is_inside (sample.cpp:268-288)
-  if (!(ph.pos.z >= R(0)) || (p.top_finite && ph.pos.z >= p.z_top) || !isfinite(ph.pos.x) ||
+  if (!(ph.pos.z >= R(0)) || ph.pos.z >= v.L[ph.layer].z_top || !isfinite(ph.pos.x) ||
       !isfinite(ph.pos.y) || !isfinite(ph.pos.z)) {
     ph.alive = false;
     return 0;
@@ -831,7 +868,13 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj>
+// The transport kernel.  kBatch: the photons of one launch belong to several
+// scenes, concatenated in photon-index order (scene i owns global indices
+// [prefix[i], prefix[i+1])); layers, species, sensors and tables of all
+// scenes then stay in global memory (cache resident: the index order keeps
+// only a few scenes in flight at a time) and each photon launches with its
+// own scene's laser.  Otherwise the scene constants are staged in LDS.
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
 __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
   extern __shared__ __align__(16) unsigned char smem[];
   unsigned char* q = smem;
@@ -840,40 +883,43 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
     q += (bytes + 15) & ~static_cast<size_t>(15);
     return r;
   };
-  LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(carve(sizeof(LayerDev<R>) * p.n_layers));
-  SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(carve(sizeof(SpeciesDev<R>) * p.n_species));
-  SensorDev* sSen = reinterpret_cast<SensorDev*>(carve(sizeof(SensorDev) * p.n_sensors));
-  PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
-  unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
-  LaunchConst<R>* sLC = reinterpret_cast<LaunchConst<R>*>(carve(sizeof(LaunchConst<R>)));
-  if (threadIdx.x == 0) {
-    LaunchConst<R> c;
-    c.lpos_x = p.lpos_x; c.lpos_y = p.lpos_y; c.sigma = p.sigma; c.pulse = p.pulse; c.rep = p.rep; c.t_off = p.t_off;
-    c.lpos_z = p.lpos_z;
-    for (int j = 0; j < 3; ++j) { c.ldir[j] = p.ldir[j]; c.lm[j] = p.lm[j]; c.ln[j] = p.ln[j]; }
-    for (int j = 0; j < 4; ++j) c.pol[j] = p.pol[j];
-    c.source = p.source; c.temporal = p.temporal; c.launch_layer = p.launch_layer; c.pad = 0;
-    *sLC = c;
-  }
   const int tid = threadIdx.x, nt = blockDim.x;
   auto copy = [&](void* dst, const void* src, size_t bytes) {
     const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
     uint32_t* d = reinterpret_cast<uint32_t*>(dst);
     for (int i = tid; i < static_cast<int>(bytes / 4); i += nt) d[i] = s[i];
   };
-  copy(sL, p.layers, sizeof(LayerDev<R>) * p.n_layers);
-  copy(sS, p.species, sizeof(SpeciesDev<R>) * p.n_species);
-  copy(sSen, p.sensors, sizeof(SensorDev) * p.n_sensors);
-  copy(sPl, p.planes, sizeof(PlaneDev) * p.n_planes);
+  unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
   for (int i = tid; i < kMaxSensors + 2; i += nt) sHits[i] = 0u;
   SceneView<R> v;
-  v.L = sL;
-  v.S = sS;
-  v.sens = sSen;
-  v.planes = sPl;
   v.lgrid = nullptr;
   v.lhits = sHits;
-  v.lc = sLC;
+  uint64_t* sPre = nullptr;
+  if constexpr (kBatch) {
+    sPre = reinterpret_cast<uint64_t*>(carve(sizeof(uint64_t) * (p.n_scenes + 1)));
+    for (int i = tid; i <= p.n_scenes; i += nt) sPre[i] = p.prefix[i];
+    v.lc = p.lcs;
+    v.L = p.layers;
+    v.S = p.species;
+    v.sens = p.sensors;
+    v.planes = p.planes;
+  } else {
+    LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(carve(sizeof(LayerDev<R>) * p.n_layers));
+    SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(carve(sizeof(SpeciesDev<R>) * p.n_species));
+    SensorDev* sSen = reinterpret_cast<SensorDev*>(carve(sizeof(SensorDev) * p.n_sensors));
+    PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
+    LaunchConst<R>* sLC = reinterpret_cast<LaunchConst<R>*>(carve(sizeof(LaunchConst<R>)));
+    copy(sL, p.layers, sizeof(LayerDev<R>) * p.n_layers);
+    copy(sS, p.species, sizeof(SpeciesDev<R>) * p.n_species);
+    copy(sSen, p.sensors, sizeof(SensorDev) * p.n_sensors);
+    copy(sPl, p.planes, sizeof(PlaneDev) * p.n_planes);
+    copy(sLC, p.lcs, sizeof(LaunchConst<R>));
+    v.lc = sLC;
+    v.L = sL;
+    v.S = sS;
+    v.sens = sSen;
+    v.planes = sPl;
+  }
   if constexpr (kLdsTables) {
     R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));
     uint16_t* tg = reinterpret_cast<uint16_t*>(carve(sizeof(uint16_t) * p.guide_total + 4));
@@ -920,7 +966,23 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
       }
       const uint64_t my = pool_next + static_cast<uint64_t>(__popcll(dead & below_mask));
       const bool take = !ph.alive && my < pool_end;
-      if (take) launch<R, kXo>(p, v, ph, static_cast<uint32_t>(p.offset + my));
+      if (take) {
+        if constexpr (kBatch) {
+          // scene of global index g: prefix[sc] <= g < prefix[sc + 1]
+          const uint64_t g = p.offset + my;
+          int lo = 0, hi = p.n_scenes;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sPre[mid] <= g)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          launch<R, kXo>(p, p.lcs[lo], ph, static_cast<uint32_t>(p.local_offset + (g - sPre[lo])));
+        } else {
+          launch<R, kXo>(p, *v.lc, ph, static_cast<uint32_t>(p.offset + my));
+        }
+      }
       const uint64_t nd = static_cast<uint64_t>(__popcll(dead));
       const uint64_t avail = pool_end - pool_next;
       const uint64_t used = nd < avail ? nd : avail;
@@ -965,8 +1027,9 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   unsigned* shits = reinterpret_cast<unsigned*>(smem + ((sizeof(double) * p.lds_grid + 15) & ~static_cast<size_t>(15)));
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int i = tid; i < p.lds_grid; i += nt) sgrid[i] = 0.0;
-  for (int i = tid; i < kMaxSensors + 1; i += nt) shits[i] = 0u;
+  for (int i = tid; i < p.n_sensors + 1; i += nt) shits[i] = 0u;
   __syncthreads();
+  unsigned* s_cbs = shits + p.n_sensors;
   const int64_t n_rec = static_cast<int64_t>(p.q_count[0]) < p.q_cap ? static_cast<int64_t>(p.q_count[0]) : p.q_cap;
   const int64_t c = p.q_cap;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * nt + tid; i < n_rec; i += static_cast<int64_t>(gridDim.x) * nt) {
@@ -988,6 +1051,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
 #pragma unroll
       for (int qq = 0; qq < 3 * kMaxGradLayers; ++qq) score[qq] = f[(F_SCORE + qq) * c];
     }
+    const LaunchConst<R>& lc = p.lcs[p.planes[pl].scene];
     const R ws = M<R>::sqrt(w);
     const cx<R> fm = cscale(Em, ws), fn = cscale(En, ws);
     // forward field on the lab axes (project_to_lab, detector.cpp:1048-1053)
@@ -998,102 +1062,73 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
       if (!((mask >> k) & 1)) continue;
       const int si = first + k;
       const SensorDev& sd = p.sensors[si];
-      bool counted = false;
-      if (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_PLANAR_FLUENCE) {
-        // time bin (detector.cpp:1182-1190, :787-797)
-        int t_idx = sd.kind == LMC_SENSOR_FARFIELD_CBS ? 0 : -1;
-        if (sd.g[5] > 0) {
-          const double a = static_cast<double>(arr);
-          if (a < 0 || a >= sd.g[2]) continue;
-          t_idx = static_cast<int>(a / sd.g[5]) + 1;
-          if (t_idx >= sd.n2) continue;
+      int64_t bin;
+      int t_idx;
+      if (!sensor_bin(sd, hx, hy, arr, P.s, bin, t_idx)) continue;
+      if (sd.kind == LMC_SENSOR_FARFIELD_CBS) {
+        cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
+        if (events >= 2) {
+          const int spk = iq[I_SP * c];
+          const Frame<R> P1 = frame_of(V3<R>{f[(F_M1 + 0) * c], f[(F_M1 + 1) * c], f[(F_M1 + 2) * c]},
+                                       V3<R>{f[(F_S1 + 0) * c], f[(F_S1 + 1) * c], f[(F_S1 + 2) * c]});
+          const Frame<R> Pq = frame_of(V3<R>{f[(F_MQ + 0) * c], f[(F_MQ + 1) * c], f[(F_MQ + 2) * c]},
+                                       V3<R>{f[(F_SQ + 0) * c], f[(F_SQ + 1) * c], f[(F_SQ + 2) * c]});
+          cx<R> T[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) T[kk] = {f[(F_T + 2 * kk) * c], f[(F_T + 2 * kk + 1) * c]};
+          const V3<R> d{f[(F_D + 0) * c], f[(F_D + 1) * c], f[(F_D + 2) * c]};
+          cx<R> rm, rn;
+          reverse_field(lc, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
+          const R arg =
+              lc.k0 * ((P.s.x + lc.ldir[0]) * d.x + (P.s.y + lc.ldir[1]) * d.y + (P.s.z + lc.ldir[2]) * d.z);
+          R sa, ca;
+          M<R>::sincos(arg, &sa, &ca);
+          const cx<R> pp{ca * ws, sa * ws};
+          rm = cmul(rm, pp);
+          rn = cmul(rn, pp);
+          Erx = cadd(cscale(rm, P.m.x), cscale(rn, P.n.x));
+          Ery = cadd(cscale(rm, P.m.y), cscale(rn, P.n.y));
+          atomicAdd(s_cbs, 1u);
         }
+        cbs_deposit<R, kAdj>(p, sd, sgrid, bin, t_idx, Efx, Efy, Erx, Ery, score);
+      } else if (sd.kind == LMC_SENSOR_PLANAR_FLUENCE) {
         const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
         const int64_t plane = tplane * sd.n2;
-        if (sd.kind == LMC_SENSOR_FARFIELD_CBS) {
-          const int it = static_cast<int>(exit_theta(P.s) / static_cast<R>(sd.g[3]));
-          const int jp = static_cast<int>(exit_phi(P.s) / static_cast<R>(sd.g[4]));
-          if (it < 0 || it >= sd.n0 || jp < 0 || jp >= sd.n1) continue;
-          const int64_t bin = static_cast<int64_t>(it) * sd.n1 + jp;
-          cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
-          if (events >= 2) {
-            const int spk = iq[I_SP * c];
-            const Frame<R> P1 = frame_of(V3<R>{f[(F_M1 + 0) * c], f[(F_M1 + 1) * c], f[(F_M1 + 2) * c]},
-                                         V3<R>{f[(F_S1 + 0) * c], f[(F_S1 + 1) * c], f[(F_S1 + 2) * c]});
-            const Frame<R> Pq = frame_of(V3<R>{f[(F_MQ + 0) * c], f[(F_MQ + 1) * c], f[(F_MQ + 2) * c]},
-                                         V3<R>{f[(F_SQ + 0) * c], f[(F_SQ + 1) * c], f[(F_SQ + 2) * c]});
-            cx<R> T[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) T[kk] = {f[(F_T + 2 * kk) * c], f[(F_T + 2 * kk + 1) * c]};
-            const V3<R> d{f[(F_D + 0) * c], f[(F_D + 1) * c], f[(F_D + 2) * c]};
-            cx<R> rm, rn;
-            reverse_field(p, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
-            const R arg =
-                p.k0 * ((P.s.x + p.ldir[0]) * d.x + (P.s.y + p.ldir[1]) * d.y + (P.s.z + p.ldir[2]) * d.z);
-            R sa, ca;
-            M<R>::sincos(arg, &sa, &ca);
-            const cx<R> pp{ca * ws, sa * ws};
-            rm = cmul(rm, pp);
-            rn = cmul(rn, pp);
-            Erx = cadd(cscale(rm, P.m.x), cscale(rn, P.n.x));
-            Ery = cadd(cscale(rm, P.m.y), cscale(rn, P.n.y));
-            atomicAdd(&shits[kMaxSensors], 1u);
-          }
-          cbs_deposit<R, kAdj>(p, sd, sgrid, bin, t_idx, Efx, Efy, Erx, Ery, score);
-          counted = true;
-        } else {
-          const int xi = static_cast<int>((static_cast<double>(hx) + 0.5 * sd.g[0]) / sd.g[3]);
-          const int yi = static_cast<int>((static_cast<double>(hy) + 0.5 * sd.g[1]) / sd.g[4]);
-          if (xi < 0 || xi >= sd.n0 || yi < 0 || yi >= sd.n1) continue;
-          const cx<R> Ex = cscale(Efx, R(-1));  // lab x negated (detector.cpp:812)
-          const int64_t bin = static_cast<int64_t>(xi) * sd.n1 + yi;
+        const cx<R> Ex = cscale(Efx, R(-1));  // lab x negated (detector.cpp:812)
 #pragma unroll 1
-          for (int q = 0; q < 4; ++q) {
-            const double dv = static_cast<double>(stokes(q, Ex, Efy));
-            if (t_idx >= 1) deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, dv, score);
-            deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, dv, score);
-          }
-          counted = true;
+        for (int q = 0; q < 4; ++q) {
+          const double dv = static_cast<double>(stokes(q, Ex, Efy));
+          if (t_idx >= 1) deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, dv, score);
+          deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, dv, score);
         }
       } else if (sd.kind == LMC_SENSOR_PLANAR_FIELD) {
-        const int xi = static_cast<int>((static_cast<double>(hx) + 0.5 * sd.g[0]) / sd.g[2]);
-        const int yi = static_cast<int>((static_cast<double>(hy) + 0.5 * sd.g[1]) / sd.g[3]);
-        if (xi < 0 || xi >= sd.n0 || yi < 0 || yi >= sd.n1) continue;
         double sp, cp;
-        ::sincos(static_cast<double>(p.k0) * p.qd[i], &sp, &cp);
+        ::sincos(static_cast<double>(lc.k0) * p.qd[i], &sp, &cp);
         const double wsd = ::sqrt(static_cast<double>(w));
         const double exr = static_cast<double>(Em.r * P.m.x + En.r * P.n.x);
         const double exi = static_cast<double>(Em.i * P.m.x + En.i * P.n.x);
         const double eyr = static_cast<double>(Em.r * P.m.y + En.r * P.n.y);
         const double eyi = static_cast<double>(Em.i * P.m.y + En.i * P.n.y);
         const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
-        const int64_t bin = static_cast<int64_t>(xi) * sd.n1 + yi;
         deposit<R, kAdj>(p, sd, sgrid, 2 * bin, (exr * cp - exi * sp) * wsd, score);
         deposit<R, kAdj>(p, sd, sgrid, 2 * bin + 1, (exr * sp + exi * cp) * wsd, score);
         deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin), (eyr * cp - eyi * sp) * wsd, score);
         deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin) + 1, (eyr * sp + eyi * cp) * wsd, score);
-        counted = true;
       } else if (sd.kind == LMC_SENSOR_FARFIELD_FLUENCE) {
-        const int it = static_cast<int>(exit_theta(P.s) / static_cast<R>(sd.g[2]));
-        const int jp = static_cast<int>(exit_phi(P.s) / static_cast<R>(sd.g[3]));
-        if (it < 0 || it >= sd.n0 || jp < 0 || jp >= sd.n1) continue;
         const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
-        const int64_t bin = static_cast<int64_t>(it) * sd.n1 + jp;
 #pragma unroll 1
         for (int q = 0; q < 4; ++q)
           deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, static_cast<double>(stokes(q, Efx, Efy)), score);
-        counted = true;
-      } else if (sd.kind == LMC_SENSOR_REFLECTANCE) {
+      } else {  // LMC_SENSOR_REFLECTANCE
         deposit<R, kAdj>(p, sd, sgrid, 0, static_cast<double>(w), score);
-        counted = true;
       }
-      if (counted) atomicAdd(&shits[si], 1u);
+      atomicAdd(&shits[si], 1u);
     }
   }
   __syncthreads();
   for (int i = tid; i < p.n_sensors; i += nt)
     if (shits[i]) atomicAdd(&p.hits[i], static_cast<unsigned long long>(shits[i]));
-  if (tid == 0 && shits[kMaxSensors]) atomicAdd(&p.stats[4], static_cast<unsigned long long>(shits[kMaxSensors]));
+  if (tid == 0 && *s_cbs) atomicAdd(&p.stats[4], static_cast<unsigned long long>(*s_cbs));
   for (int si = 0; si < p.n_sensors; ++si) {
     const SensorDev& sd = p.sensors[si];
     if (!sd.lds) continue;

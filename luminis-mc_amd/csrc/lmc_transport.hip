@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -96,7 +97,7 @@ template <class T> struct DBuf {
   ~DBuf() { release(); }
 };
 
-// Precision-specific device copy of a scene's tables.
+// Precision-specific device copy of a scene set's tables.
 template <class R> struct DevTables {
   bool ready = false;
   DBuf<LayerDev<R>> layers;
@@ -104,46 +105,70 @@ template <class R> struct DevTables {
   DBuf<R> cdf, sel;
   DBuf<uint16_t> guide;
   DBuf<STab<R>> stab;
+  DBuf<LaunchConst<R>> lcs;
   int cdf_total = 0, guide_total = 0, stab_total = 0, sel_total = 0;
   size_t table_bytes = 0;
 };
 
+// Per-scene bookkeeping of a scene set.
+struct SceneInfo {
+  int layer_base = 0, n_layers = 0;
+  int species_base = 0, n_species = 0;
+  int plane_first = 0, plane_count = 0;
+  int launch_layer = 0;        // absolute
+  int64_t out_base = 0, n_out = 0;
+  int n_sensors = 0;
+  int n_single_planes = 0;     // planes a photon crosses at most once (records <= 1)
+  int n_multi_planes = 0;      // interior non-absorbing planes (records <= max_events + 2)
+  lmc_laser_desc laser{};
+};
+
 }  // namespace
 
+// A scene set: one scene (lmc_scene) or many independent scenes run in one
+// launch (lmc_batch).  Layers, species, tables, sensors and accumulators of
+// all scenes are concatenated with absolute indices.
 struct lmc_scene {
-  // host copies of the descriptor
+  // merged host copies of the descriptors
   std::vector<lmc_layer_desc> layers;
   std::vector<lmc_species_desc> species;
-  std::vector<lmc_sensor_desc> sensors;
   std::vector<double> tables;
-  lmc_laser_desc laser{};
-  double n_medium = 1.0;
+  std::vector<int> layer_scene;
+  std::vector<SceneInfo> scenes;
   int64_t n_out = 0;
-  int launch_layer = 0;
   // device
   DevTables<float> t32;
   DevTables<double> t64;
   DBuf<SensorDev> sensors_dev;
   DBuf<PlaneDev> planes_dev;
   std::vector<SensorDev> sensors_host;
+  std::vector<int> sensor_scene;
   std::vector<PlaneDev> planes_host;
-  std::vector<int> plane_single;  // 1: at most one crossing record per photon
   int lds_grid = 0;
   DBuf<double> out;
   DBuf<unsigned long long> hits;
   DBuf<unsigned long long> counter;  // [0] work counter, [1..5] stats
   DBuf<double> grad;
-  DBuf<unsigned char> queue;        // CBS record queue (SoA)
+  DBuf<unsigned char> queue;        // crossing record queue (SoA)
   DBuf<unsigned long long> qcount;  // [0] written, [1] overflowed
+  DBuf<uint64_t> prefix;            // batched runs: photon-index prefix over scenes
   int64_t q_cap = 0;
   size_t q_rec_bytes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  // captured batched run, replayed while its key is unchanged
+  hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> graph_key;
+};
+
+struct lmc_batch {
+  lmc_scene set;
 };
 
 namespace {
 
 constexpr int kGuideN = 4096;
+constexpr int kMaxBatchScenes = 4096;
 
 template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if (T.ready) return LMC_OK;
@@ -197,9 +222,13 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
       stab.push_back(e);
     }
   }
+  if (cdf.size() > 0x7FFFFFFFull || stab.size() > 0x7FFFFFFFull || guide.size() > 0x7FFFFFFFull)
+    return fail(LMC_ERR_INVALID, "tables too large");
   std::vector<LayerDev<R>> L(s->layers.size());
   for (size_t i = 0; i < s->layers.size(); ++i) {
     const lmc_layer_desc& d = s->layers[i];
+    const SceneInfo& sc = s->scenes[s->layer_scene[i]];
+    const int li = static_cast<int>(i) - sc.layer_base;
     LayerDev<R>& o = L[i];
     std::memset(&o, 0, sizeof(o));
     o.z_min = static_cast<R>(d.z_min);
@@ -213,12 +242,40 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.sp_first = d.species_first;
     o.sp_count = d.species_count;
     o.sel_off = static_cast<int32_t>(sel.size());
+    o.nbr = (li + 1 < sc.n_layers ? 1 : 0) | (li > 0 ? 2 : 0);
+    o.pl_first = sc.plane_first;
+    o.pl_count = sc.plane_count;
+    o.z_top = static_cast<R>(s->layers[sc.layer_base + sc.n_layers - 1].z_max);  // +inf when semi-infinite
     if (d.species_count > 1)
       for (int j = 0; j < d.species_count; ++j) sel.push_back(static_cast<R>(s->tables[d.select_offset + j]));
+  }
+  std::vector<LaunchConst<R>> lc(s->scenes.size());
+  for (size_t k = 0; k < s->scenes.size(); ++k) {
+    const lmc_laser_desc& Ls = s->scenes[k].laser;
+    LaunchConst<R>& c = lc[k];
+    std::memset(&c, 0, sizeof(c));
+    c.lpos_x = Ls.pos[0];
+    c.lpos_y = Ls.pos[1];
+    c.lpos_z = static_cast<R>(Ls.pos[2]);
+    for (int j = 0; j < 3; ++j) {
+      c.ldir[j] = static_cast<R>(Ls.dir[j]);
+      c.lm[j] = static_cast<R>(Ls.m[j]);
+      c.ln[j] = static_cast<R>(Ls.n[j]);
+    }
+    for (int j = 0; j < 4; ++j) c.pol[j] = static_cast<R>(Ls.pol[j]);
+    c.k0 = static_cast<R>(2.0 * M_PI / Ls.wavelength);
+    c.sigma = Ls.sigma;
+    c.source = Ls.source;
+    c.temporal = Ls.temporal;
+    c.pulse = Ls.pulse_duration;
+    c.rep = Ls.repetition_rate;
+    c.t_off = Ls.time_offset;
+    c.launch_layer = s->scenes[k].launch_layer;
   }
   int rc;
   if ((rc = T.layers.upload(L.data(), L.size()))) return rc;
   if ((rc = T.species.upload(sp.data(), sp.size()))) return rc;
+  if ((rc = T.lcs.upload(lc.data(), lc.size()))) return rc;
   if (cdf.empty()) cdf.push_back(R(1));
   if (guide.empty()) guide.assign(2, 0);
   if (sel.empty()) sel.push_back(R(1));
@@ -297,26 +354,6 @@ int launch_layer_of(const lmc_scene_desc* d, double z) {
   return idx;
 }
 
-template <class R> void fill_laser(const lmc_scene* s, KParams<R>& p) {
-  const lmc_laser_desc& L = s->laser;
-  p.lpos_x = L.pos[0];
-  p.lpos_y = L.pos[1];
-  p.lpos_z = static_cast<R>(L.pos[2]);
-  for (int j = 0; j < 3; ++j) {
-    p.ldir[j] = static_cast<R>(L.dir[j]);
-    p.lm[j] = static_cast<R>(L.m[j]);
-    p.ln[j] = static_cast<R>(L.n[j]);
-  }
-  for (int j = 0; j < 4; ++j) p.pol[j] = static_cast<R>(L.pol[j]);
-  p.k0 = static_cast<R>(2.0 * M_PI / L.wavelength);
-  p.sigma = L.sigma;
-  p.source = L.source;
-  p.temporal = L.temporal;
-  p.pulse = L.pulse_duration;
-  p.rep = L.repetition_rate;
-  p.t_off = L.time_offset;
-}
-
 template <class K> int raise_lds_limit(K kfn, size_t lds) {
   if (lds > 65536)
     LMC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kfn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -324,16 +361,32 @@ template <class K> int raise_lds_limit(K kfn, size_t lds) {
   return LMC_OK;
 }
 
-// Crossing records per transport launch.  A run is split into chunks of
+// Crossing records per transport launch.  A run is split into launches of
 // q_cap / (worst-case records per photon) photons so the queue never fills:
 // one record per plane a photon can cross only when leaving the sample (or
-// whose sensors all absorb), max_events + 2 for other interior planes.
-constexpr int64_t kQueueRecords = int64_t(1) << 23;
+// whose sensors all absorb), max_events + 2 for other interior planes.  Large
+// launches keep the drain tail of the persistent kernel (the last, longest
+// paths) short against the launch; the queue takes at most a quarter of the
+// free device memory.
+constexpr int64_t kQueueRecords = int64_t(1) << 26;
+
+void drop_graph(lmc_scene* s) {
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  s->graph = nullptr;
+  s->graph_key.clear();
+}
 
 template <class R> int ensure_queue(lmc_scene* s, int64_t cap) {
-  cap = (cap + 1) & ~int64_t(1);
   const size_t rec = sizeof(R) * kRecF + sizeof(double) + sizeof(int32_t) * kRecI;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    const int64_t fit = static_cast<int64_t>((free_b + (s->queue.p ? s->queue.n : 0)) / 4 / rec);
+    cap = std::min(cap, std::max<int64_t>(fit, 1 << 16));
+  }
+  cap = (cap + 1) & ~int64_t(1);
   if (s->queue.p && s->q_cap >= cap && s->q_rec_bytes == rec) return LMC_OK;
+  drop_graph(s);  // a captured graph holds the old queue's address
+  s->queue.release();
   int rc = s->queue.alloc(static_cast<size_t>(cap) * rec + 256);
   if (rc) return rc;
   s->q_cap = cap;
@@ -341,8 +394,21 @@ template <class R> int ensure_queue(lmc_scene* s, int64_t cap) {
   return LMC_OK;
 }
 
+int64_t records_per_photon(const SceneInfo& sc, int max_events) {
+  return sc.n_single_planes + static_cast<int64_t>(sc.n_multi_planes) * (static_cast<int64_t>(max_events) + 2);
+}
+
+
+// One run over a scene set.  budgets == nullptr: the set's single scene, photons
+// [photon_offset, photon_offset + n_photons).  Otherwise budgets[i] photons of
+// scene i, photon indices photon_offset + [0, budgets[i]) within each scene, in
+// launches over the concatenated index space; use_graph captures the launch
+// sequence once into a hipGraph and replays it while (budgets, run) repeat.
 template <class R, bool kXo, bool kAdj>
-int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st) {
+int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph,
+             lmc_stats* st) {
+  const bool batch = budgets != nullptr;
+  const int n_sc = static_cast<int>(s->scenes.size());
   KParams<R> p;
   std::memset(&p, 0, sizeof(p));
   p.layers = T.layers.p;
@@ -353,60 +419,85 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st
   p.sel = T.sel.p;
   p.sensors = s->sensors_dev.p;
   p.planes = s->planes_dev.p;
+  p.lcs = T.lcs.p;
   p.n_layers = static_cast<int32_t>(s->layers.size());
   p.n_species = static_cast<int32_t>(s->species.size());
-  p.n_sensors = static_cast<int32_t>(s->sensors.size());
+  p.n_sensors = static_cast<int32_t>(s->sensors_host.size());
   p.n_planes = static_cast<int32_t>(s->planes_host.size());
+  p.n_scenes = n_sc;
   p.cdf_total = T.cdf_total;
   p.guide_total = T.guide_total;
   p.stab_total = T.stab_total;
   p.sel_total = T.sel_total;
   p.lds_grid = s->lds_grid;
-  p.launch_layer = s->launch_layer;
-  fill_laser(s, p);
   p.key0 = static_cast<uint32_t>(r->seed);
   p.key1 = static_cast<uint32_t>(r->seed >> 32);
   p.max_events = r->max_events;
   p.claim = r->claim > 0 ? r->claim : 256;
-  const double ztop = s->layers.back().z_max;
-  p.top_finite = std::isinf(ztop) ? 0 : 1;
-  p.z_top = static_cast<R>(p.top_finite ? ztop : 0.0);
   p.out = s->out.p;
   p.hits = s->hits.p;
   p.counter = s->counter.p;
   p.stats = s->counter.p + 1;
   p.grad = s->grad.p;
   p.n_out = s->n_out;
+  uint64_t total = r->n_photons;
+  int64_t per_photon = 0;
+  std::vector<uint64_t> pre;
+  if (batch) {
+    pre.assign(n_sc + 1, 0);
+    for (int k = 0; k < n_sc; ++k) {
+      if (budgets[k] > 0 && r->photon_offset + budgets[k] > (uint64_t(1) << 32))
+        return fail(LMC_ERR_INVALID, "batched runs index photons of a scene below 2^32");
+      pre[k + 1] = pre[k] + budgets[k];
+      if (budgets[k] > 0) per_photon = std::max(per_photon, records_per_photon(s->scenes[k], r->max_events));
+    }
+    total = pre[n_sc];
+    if (s->prefix.n != pre.size()) {  // a captured graph holds the buffer's address
+      drop_graph(s);
+      int rc = s->prefix.alloc(pre.size());
+      if (rc) return rc;
+    }
+    LMC_HIP(hipMemcpy(s->prefix.p, pre.data(), sizeof(uint64_t) * pre.size(), hipMemcpyHostToDevice));
+    p.prefix = s->prefix.p;
+    p.local_offset = r->photon_offset;
+  } else {
+    per_photon = records_per_photon(s->scenes[0], r->max_events);
+  }
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (total == 0) return LMC_OK;
   size_t base_lds = 0;
   auto add = [&](size_t b) { base_lds += (b + 15) & ~static_cast<size_t>(15); };
-  add(sizeof(LayerDev<R>) * p.n_layers);
-  add(sizeof(SpeciesDev<R>) * p.n_species);
-  add(sizeof(SensorDev) * p.n_sensors);
-  add(sizeof(PlaneDev) * p.n_planes);
   add(sizeof(unsigned) * (kMaxSensors + 2));
-  add(sizeof(LaunchConst<R>));
+  if (batch) {
+    add(sizeof(uint64_t) * (n_sc + 1));
+  } else {
+    add(sizeof(LayerDev<R>) * p.n_layers);
+    add(sizeof(SpeciesDev<R>) * p.n_species);
+    add(sizeof(SensorDev) * p.n_sensors);
+    add(sizeof(PlaneDev) * p.n_planes);
+    add(sizeof(LaunchConst<R>));
+  }
   const size_t tab_lds = ((sizeof(R) * T.cdf_total + 15) & ~15ull) +
                          ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
                          ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
   // Launch shape: 16 resident waves per CU (4 per SIMD at <= 128 VGPRs).  The
   // tables go to LDS when two 512-thread workgroups per CU can hold them;
   // otherwise 256-thread workgroups read them through the caches.
-  const bool fit512 = 2 * (base_lds + tab_lds) <= g_max_lds_cu && base_lds + tab_lds <= g_max_lds_block;
+  const bool fit512 = !batch && 2 * (base_lds + tab_lds) <= g_max_lds_cu && base_lds + tab_lds <= g_max_lds_block;
   int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : 256);
   if (block % 64 != 0 || block > 512) return fail(LMC_ERR_INVALID, "block_size must be a multiple of 64, <= 512");
   const int bpc = r->blocks_per_cu > 0 ? r->blocks_per_cu : std::max(1, 1024 / block);
   const int grid = g_num_cu * bpc;
-  const bool lds_tables = (base_lds + tab_lds) * static_cast<size_t>(std::min(bpc, 1024 / block)) <= g_max_lds_cu &&
+  const bool lds_tables = !batch &&
+                          (base_lds + tab_lds) * static_cast<size_t>(std::min(bpc, 1024 / block)) <= g_max_lds_cu &&
                           base_lds + tab_lds <= g_max_lds_block;
   const size_t lds = lds_tables ? base_lds + tab_lds : base_lds;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   const bool track = r->track_reverse_paths != 0;
-  int64_t per_photon = 0;
-  for (size_t i = 0; i < s->planes_host.size(); ++i)
-    per_photon += s->plane_single[i] ? 1 : static_cast<int64_t>(r->max_events) + 2;
-  int64_t chunk = static_cast<int64_t>(r->n_photons);
+  int64_t chunk = static_cast<int64_t>(total);
   if (per_photon > 0) {
-    const int64_t cap = std::min<int64_t>(kQueueRecords, static_cast<int64_t>(r->n_photons) * per_photon);
+    const int64_t want = static_cast<int64_t>(std::min<uint64_t>(total, uint64_t(1) << 40)) * per_photon;
+    const int64_t cap = std::min<int64_t>(kQueueRecords, want);
     int rc = ensure_queue<R>(s, cap);
     if (rc) return rc;
     chunk = std::max<int64_t>(1, s->q_cap / per_photon);
@@ -417,53 +508,116 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st
   }
   p.q_count = s->qcount.p;
   void (*kt)(KParams<R>) = nullptr;
-  if (track)
-    kt = lds_tables ? k_transport<R, kXo, true, true, kAdj> : k_transport<R, kXo, true, false, kAdj>;
+  if (batch)
+    kt = track ? k_transport<R, kXo, true, false, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
+  else if (track)
+    kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, false> : k_transport<R, kXo, true, false, kAdj, false>;
   else
-    kt = lds_tables ? k_transport<R, kXo, false, true, kAdj> : k_transport<R, kXo, false, false, kAdj>;
+    kt = lds_tables ? k_transport<R, kXo, false, true, kAdj, false> : k_transport<R, kXo, false, false, kAdj, false>;
   int rc = raise_lds_limit(kt, lds);
   if (rc) return rc;
   void (*kr)(KParams<R>) = k_crossings<R, kAdj>;
-  const size_t lds_r = ((sizeof(double) * p.lds_grid + 15) & ~15ull) + sizeof(unsigned) * (kMaxSensors + 1);
+  const size_t lds_r = ((sizeof(double) * p.lds_grid + 15) & ~15ull) + sizeof(unsigned) * (p.n_sensors + 1);
+  if (lds_r > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "too many sensors for the sensor kernel");
   if ((rc = raise_lds_limit(kr, lds_r))) return rc;
+  // the launch sequence: [zero work counter + record count, transport, sensors] per launch
+  struct Launch { uint64_t first, n; };
+  std::vector<Launch> seq;
+  for (uint64_t done = 0; done < total;) {
+    uint64_t n = std::min<uint64_t>(static_cast<uint64_t>(chunk), total - done);
+    if (!batch) {  // photon indices of one launch share their high word (a uniform register)
+      const uint64_t first = r->photon_offset + done;
+      const uint64_t to_wrap = (uint64_t(1) << 32) - (first & 0xFFFFFFFFull);
+      if (n > to_wrap) n = to_wrap;
+    }
+    seq.push_back({done, n});
+    done += n;
+  }
+  auto enqueue = [&](const Launch& L) -> int {
+    p.n_photons = L.n;
+    if (batch) {
+      p.offset = L.first;
+      p.idx_hi = 0;
+    } else {
+      p.offset = r->photon_offset + L.first;
+      p.idx_hi = static_cast<uint32_t>(p.offset >> 32);
+    }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->counter.p, 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->qcount.p, 1);
+    LMC_HIP(hipGetLastError());
+    return LMC_OK;
+  };
+  // stats and the overflow flag start at zero once per run
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->counter.p, 8);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->qcount.p, 2);
   LMC_HIP(hipGetLastError());
   float ms_t = 0.f, ms_r = 0.f;
-  uint64_t done = 0, launches = 0;
-  while (done < r->n_photons) {
-    ++launches;
-    uint64_t n = std::min<uint64_t>(static_cast<uint64_t>(chunk), r->n_photons - done);
-    // photon indices of one launch share their high word (a uniform register)
-    const uint64_t first = r->photon_offset + done;
-    const uint64_t to_wrap = (uint64_t(1) << 32) - (first & 0xFFFFFFFFull);
-    if (n > to_wrap) n = to_wrap;
-    p.n_photons = n;
-    p.offset = first;
-    p.idx_hi = static_cast<uint32_t>(first >> 32);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->counter.p, 1);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->qcount.p, 2);
-    LMC_HIP(hipGetLastError());
-    LMC_HIP(hipEventRecord(s->ev0, s->stream));
-    hipLaunchKernelGGL(kt, dim3(grid), dim3(block), lds, s->stream, p);
-    LMC_HIP(hipGetLastError());
-    LMC_HIP(hipEventRecord(s->ev1, s->stream));
-    if (p.q_cap > 0) {
-      hipLaunchKernelGGL(kr, dim3(g_num_cu * 4), dim3(256), lds_r, s->stream, p);
-      LMC_HIP(hipGetLastError());
+  if (use_graph) {
+    std::vector<uint64_t> key(pre.begin(), pre.end());
+    const uint64_t kv[] = {r->photon_offset, r->seed, static_cast<uint64_t>(r->max_events),
+                           static_cast<uint64_t>(track), static_cast<uint64_t>(block), static_cast<uint64_t>(bpc),
+                           static_cast<uint64_t>(p.claim), static_cast<uint64_t>(sizeof(R)), static_cast<uint64_t>(kXo),
+                           static_cast<uint64_t>(kAdj), static_cast<uint64_t>(r->n_photons)};
+    key.insert(key.end(), kv, kv + sizeof(kv) / sizeof(kv[0]));
+    if (!s->graph || s->graph_key != key) {
+      drop_graph(s);
+      LMC_HIP(hipStreamSynchronize(s->stream));
+      LMC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      int crc = LMC_OK;
+      for (const Launch& L : seq) {
+        if ((crc = enqueue(L))) break;
+        hipLaunchKernelGGL(kt, dim3(grid), dim3(block), lds, s->stream, p);
+        if (p.q_cap > 0) hipLaunchKernelGGL(kr, dim3(g_num_cu * 4), dim3(256), lds_r, s->stream, p);
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(s->stream, &g);
+      if (crc) {
+        if (g) (void)hipGraphDestroy(g);
+        return crc;
+      }
+      if (ce != hipSuccess) return fail(LMC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        s->graph = nullptr;
+        return fail(LMC_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      }
+      s->graph_key = key;
     }
-    LMC_HIP(hipEventRecord(s->ev2, s->stream));
-    LMC_HIP(hipEventSynchronize(s->ev2));
-    float a = 0.f, b = 0.f;
-    LMC_HIP(hipEventElapsedTime(&a, s->ev0, s->ev1));
-    LMC_HIP(hipEventElapsedTime(&b, s->ev1, s->ev2));
-    ms_t += a;
-    ms_r += b;
+    LMC_HIP(hipEventRecord(s->ev0, s->stream));
+    LMC_HIP(hipGraphLaunch(s->graph, s->stream));
+    LMC_HIP(hipEventRecord(s->ev1, s->stream));
+    LMC_HIP(hipEventSynchronize(s->ev1));
+    LMC_HIP(hipEventElapsedTime(&ms_t, s->ev0, s->ev1));
     if (p.q_cap > 0) {
       unsigned long long qc[2];
       LMC_HIP(hipMemcpy(qc, s->qcount.p, sizeof(qc), hipMemcpyDeviceToHost));
-      if (qc[1] != 0) return fail(LMC_ERR_DEVICE, "CBS record queue overflowed");
+      if (qc[1] != 0) return fail(LMC_ERR_DEVICE, "crossing record queue overflowed");
     }
-    done += n;
+  } else {
+    for (const Launch& L : seq) {
+      if ((rc = enqueue(L))) return rc;
+      LMC_HIP(hipEventRecord(s->ev0, s->stream));
+      hipLaunchKernelGGL(kt, dim3(grid), dim3(block), lds, s->stream, p);
+      LMC_HIP(hipGetLastError());
+      LMC_HIP(hipEventRecord(s->ev1, s->stream));
+      if (p.q_cap > 0) {
+        hipLaunchKernelGGL(kr, dim3(g_num_cu * 4), dim3(256), lds_r, s->stream, p);
+        LMC_HIP(hipGetLastError());
+      }
+      LMC_HIP(hipEventRecord(s->ev2, s->stream));
+      LMC_HIP(hipEventSynchronize(s->ev2));
+      float a = 0.f, b = 0.f;
+      LMC_HIP(hipEventElapsedTime(&a, s->ev0, s->ev1));
+      LMC_HIP(hipEventElapsedTime(&b, s->ev1, s->ev2));
+      ms_t += a;
+      ms_r += b;
+      if (p.q_cap > 0) {
+        unsigned long long qc[2];
+        LMC_HIP(hipMemcpy(qc, s->qcount.p, sizeof(qc), hipMemcpyDeviceToHost));
+        if (qc[1] != 0) return fail(LMC_ERR_DEVICE, "crossing record queue overflowed");
+      }
+    }
   }
   if (st) {
     unsigned long long c[8];
@@ -474,17 +628,18 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, lmc_stats* st
     st->crossings = c[4];
     st->cbs_detections = c[5];
     st->kernel_ms = ms_t + ms_r;
-    st->transport_ms = ms_t;
+    st->transport_ms = ms_t;  // graph runs: the whole graph
     st->sensor_ms = ms_r;
-    st->transport_launches = launches;
+    st->transport_launches = seq.size();
   }
   return LMC_OK;
 }
 
-template <bool kAdj> int run_dispatch(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) {
+template <bool kAdj>
+int run_dispatch(lmc_scene* s, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph, lmc_stats* st) {
   if (!s || !r) return fail(LMC_ERR_INVALID, "null argument");
   if (r->max_events < 0) return fail(LMC_ERR_INVALID, "max_events must be >= 0");
-  if (r->n_photons == 0) {
+  if (!budgets && r->n_photons == 0) {
     if (st) std::memset(st, 0, sizeof(*st));
     return LMC_OK;
   }
@@ -492,15 +647,233 @@ template <bool kAdj> int run_dispatch(lmc_scene* s, const lmc_run_desc* r, lmc_s
     if (r->rng != LMC_RNG_PHILOX) return fail(LMC_ERR_UNSUPPORTED, "FP32 transport supports the Philox stream only");
     int rc = build_tables<float>(s, s->t32);
     if (rc) return rc;
-    return run_impl<float, false, kAdj>(s, s->t32, r, st);
+    return run_impl<float, false, kAdj>(s, s->t32, r, budgets, use_graph, st);
   }
   if (r->precision == LMC_PREC_FP64) {
     int rc = build_tables<double>(s, s->t64);
     if (rc) return rc;
-    if (r->rng == LMC_RNG_XOSHIRO) return run_impl<double, true, kAdj>(s, s->t64, r, st);
-    return run_impl<double, false, kAdj>(s, s->t64, r, st);
+    if (r->rng == LMC_RNG_XOSHIRO) return run_impl<double, true, kAdj>(s, s->t64, r, budgets, use_graph, st);
+    return run_impl<double, false, kAdj>(s, s->t64, r, budgets, use_graph, st);
   }
   return fail(LMC_ERR_INVALID, "unknown precision");
+}
+
+// Conservative bounds of a sensor's grid for the queueing test in the
+// transport kernel: a crossing outside them cannot be binned by the sensor
+// kernel (which applies the exact binning), so it is not queued.  Margins
+// cover FP32 rounding of the compared quantities.
+void queue_bounds(SensorDev& o) {
+  const float inf = std::numeric_limits<float>::infinity();
+  o.q_cmin = -inf;
+  o.q_x0 = o.q_y0 = o.q_t0 = -inf;
+  o.q_x1 = o.q_y1 = o.q_t1 = inf;
+  auto cos_floor = [](double theta_max) -> float {
+    if (!(theta_max < 3.0)) return -std::numeric_limits<float>::infinity();
+    return static_cast<float>(std::cos(theta_max) - 1e-5);
+  };
+  switch (o.kind) {
+    case LMC_SENSOR_FARFIELD_CBS:
+      o.q_cmin = cos_floor(o.n0 * o.g[3]);
+      if (o.g[5] > 0) {
+        o.q_t0 = -1e-6f;
+        o.q_t1 = static_cast<float>(o.g[2] * (1 + 1e-5) + 1e-6);
+      }
+      break;
+    case LMC_SENSOR_FARFIELD_FLUENCE:
+      o.q_cmin = cos_floor(o.n0 * o.g[2]);
+      break;
+    case LMC_SENSOR_PLANAR_FLUENCE:
+    case LMC_SENSOR_PLANAR_FIELD: {
+      const bool flu = o.kind == LMC_SENSOR_PLANAR_FLUENCE;
+      const double wx = o.n0 * (flu ? o.g[3] : o.g[2]), wy = o.n1 * (flu ? o.g[4] : o.g[3]);
+      const double mx = 1e-5 * (std::fabs(wx) + std::fabs(o.g[0])) + 1e-6;
+      const double my = 1e-5 * (std::fabs(wy) + std::fabs(o.g[1])) + 1e-6;
+      // the pixel index truncates toward zero: (-1, 0) still lands in pixel 0
+      const double dx = flu ? o.g[3] : o.g[2], dy = flu ? o.g[4] : o.g[3];
+      o.q_x0 = static_cast<float>(-0.5 * o.g[0] - std::fabs(dx) - mx);
+      o.q_x1 = static_cast<float>(-0.5 * o.g[0] + wx + mx);
+      o.q_y0 = static_cast<float>(-0.5 * o.g[1] - std::fabs(dy) - my);
+      o.q_y1 = static_cast<float>(-0.5 * o.g[1] + wy + my);
+      if (flu && o.g[5] > 0) {
+        o.q_t0 = -1e-6f;
+        o.q_t1 = static_cast<float>(o.g[2] * (1 + 1e-5) + 1e-6);
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Append one validated scene to a set: absolute species, table, sensor and
+// accumulator offsets; sensors grouped into planes with the reference's 1e-9
+// key tolerance (detector.hpp:46-49), ascending z.
+int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
+  int rc = validate(d);
+  if (rc) return rc;
+  const int ll = launch_layer_of(d, d->laser.pos[2]);
+  if (ll < 0) return fail(LMC_ERR_INVALID, "laser position lies outside the sample");
+  SceneInfo sc;
+  sc.layer_base = static_cast<int>(s->layers.size());
+  sc.n_layers = d->n_layers;
+  sc.species_base = static_cast<int>(s->species.size());
+  sc.n_species = d->n_species;
+  sc.launch_layer = sc.layer_base + ll;
+  sc.out_base = s->n_out;
+  sc.n_out = d->n_out;
+  sc.n_sensors = d->n_sensors;
+  sc.laser = d->laser;
+  const int64_t tbase = static_cast<int64_t>(s->tables.size());
+  const int scene_idx = static_cast<int>(s->scenes.size());
+  if (d->n_tables) s->tables.insert(s->tables.end(), d->tables, d->tables + d->n_tables);
+  for (int i = 0; i < d->n_layers; ++i) {
+    lmc_layer_desc L = d->layers[i];
+    L.species_first += sc.species_base;
+    if (L.species_count > 1) L.select_offset += tbase;
+    s->layers.push_back(L);
+    s->layer_scene.push_back(scene_idx);
+  }
+  for (int i = 0; i < d->n_species; ++i) {
+    lmc_species_desc S = d->species[i];
+    S.cdf_offset += tbase;
+    S.s1_offset += tbase;
+    S.s2_offset += tbase;
+    s->species.push_back(S);
+  }
+  std::vector<double> zs;
+  for (int i = 0; i < d->n_sensors; ++i) {
+    const double z = d->sensors[i].z;
+    bool found = false;
+    for (double zz : zs)
+      if (!(z < zz - 1e-9) && !(zz < z - 1e-9)) found = true;
+    if (!found) zs.push_back(z);
+  }
+  std::sort(zs.begin(), zs.end());
+  sc.plane_first = static_cast<int>(s->planes_host.size());
+  sc.plane_count = static_cast<int>(zs.size());
+  const double ztop_s = d->layers[d->n_layers - 1].z_max;
+  for (size_t pl = 0; pl < zs.size(); ++pl) {
+    PlaneDev P;
+    std::memset(&P, 0, sizeof(P));
+    P.z = zs[pl];
+    P.first = static_cast<int32_t>(s->sensors_host.size());
+    P.count = 0;
+    P.scene = scene_idx;
+    bool all_absorb = true;
+    for (int i = 0; i < d->n_sensors; ++i) {
+      const lmc_sensor_desc& sd = d->sensors[i];
+      if ((sd.z < zs[pl] - 1e-9) || (zs[pl] < sd.z - 1e-9)) continue;
+      SensorDev o;
+      std::memset(&o, 0, sizeof(o));
+      o.kind = sd.kind;
+      o.absorb = sd.absorb;
+      o.plane = static_cast<int32_t>(sc.plane_first + pl);
+      o.id = i;
+      o.f_dir_en = sd.f_dir_en;
+      o.f_dir = sd.f_dir;
+      o.f_ev_en = sd.f_ev_en;
+      o.f_ev_min = sd.f_ev_min;
+      o.f_ev_max = sd.f_ev_max;
+      o.f_th_en = sd.f_th_en;
+      o.f_ph_en = sd.f_ph_en;
+      o.f_pos_en = sd.f_pos_en;
+      o.cos_th_min = static_cast<float>(sd.cos_th_min);
+      o.cos_th_max = static_cast<float>(sd.cos_th_max);
+      o.ph_min = static_cast<float>(sd.ph_min);
+      o.ph_max = static_cast<float>(sd.ph_max);
+      o.x_min = static_cast<float>(sd.x_min);
+      o.x_max = static_cast<float>(sd.x_max);
+      o.y_min = static_cast<float>(sd.y_min);
+      o.y_max = static_cast<float>(sd.y_max);
+      for (int j = 0; j < 8; ++j) o.g[j] = sd.g[j];
+      o.n0 = sd.n[0];
+      o.n1 = sd.n[1];
+      o.n2 = sd.n[2];
+      o.out_off = sc.out_base + sd.out_offset;
+      queue_bounds(o);
+      // privatise small angular grids per workgroup of the sensor kernel (<= 48 KiB)
+      const bool small = privatise &&
+                         (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_FARFIELD_FLUENCE ||
+                          sd.kind == LMC_SENSOR_REFLECTANCE) &&
+                         (s->lds_grid + sd.out_len) * 8 <= 49152;
+      if (small) {
+        o.lds = 1;
+        o.lds_off = s->lds_grid;
+        s->lds_grid += static_cast<int>(sd.out_len);
+      }
+      s->sensors_host.push_back(o);
+      s->sensor_scene.push_back(scene_idx);
+      P.count++;
+      if (sd.kind == LMC_SENSOR_FARFIELD_CBS) P.has_cbs = 1;
+      if (!sd.absorb) all_absorb = false;
+    }
+    // a plane on or outside the stack's faces is crossed only by a photon
+    // leaving the sample; an interior plane whose sensors all absorb records
+    // at most one crossing as well
+    const bool boundary = zs[pl] <= 0.0 || (!std::isinf(ztop_s) && zs[pl] >= ztop_s);
+    if (boundary || all_absorb)
+      sc.n_single_planes++;
+    else
+      sc.n_multi_planes++;
+    s->planes_host.push_back(P);
+  }
+  s->n_out += d->n_out;
+  s->scenes.push_back(sc);
+  return LMC_OK;
+}
+
+int finish_set(lmc_scene* s) {
+  int rc = s->sensors_dev.upload(s->sensors_host.data(), s->sensors_host.size());
+  if (!rc) rc = s->planes_dev.upload(s->planes_host.data(), s->planes_host.size());
+  if (!rc) rc = s->out.alloc(std::max<int64_t>(1, s->n_out));
+  if (!rc) rc = s->hits.alloc(std::max<size_t>(1, s->sensors_host.size()));
+  if (!rc) rc = s->counter.alloc(8);
+  if (!rc) rc = s->qcount.alloc(2);
+  if (rc) return rc;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
+      hipEventCreate(&s->ev2) != hipSuccess)
+    return fail(LMC_ERR_DEVICE, "stream/event creation failed");
+  return LMC_OK;
+}
+
+void destroy_set(lmc_scene* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_graph(s);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->ev2) (void)hipEventDestroy(s->ev2);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->ev0 = s->ev1 = s->ev2 = nullptr;
+  s->stream = nullptr;
+}
+
+int zero_set(lmc_scene* s) {
+  LMC_HIP(hipMemsetAsync(s->out.p, 0, sizeof(double) * s->out.n, s->stream));
+  LMC_HIP(hipMemsetAsync(s->hits.p, 0, sizeof(unsigned long long) * s->hits.n, s->stream));
+  if (s->grad.p) LMC_HIP(hipMemsetAsync(s->grad.p, 0, sizeof(double) * s->grad.n, s->stream));
+  LMC_HIP(hipStreamSynchronize(s->stream));
+  return LMC_OK;
+}
+
+// Add scene k's accumulators and hit counters into caller buffers.
+int read_scene(lmc_scene* s, int k, double* out, int64_t n, uint64_t* hits) {
+  const SceneInfo& sc = s->scenes[k];
+  if (out) {
+    if (n != sc.n_out) return fail(LMC_ERR_INVALID, "read size does not match the scene's accumulators");
+    std::vector<double> tmp(sc.n_out);
+    if (sc.n_out)
+      LMC_HIP(hipMemcpy(tmp.data(), s->out.p + sc.out_base, sizeof(double) * sc.n_out, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < sc.n_out; ++i) out[i] += tmp[i];
+  }
+  if (hits) {
+    std::vector<unsigned long long> h(s->hits.n);
+    LMC_HIP(hipMemcpy(h.data(), s->hits.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < s->sensors_host.size(); ++j)
+      if (s->sensor_scene[j] == k) hits[s->sensors_host[j].id] += h[j];
+  }
+  return LMC_OK;
 }
 
 }  // namespace
@@ -538,108 +911,13 @@ int lmc_scene_create(const lmc_scene_desc* d, lmc_scene** out) {
   int rc = validate(d);
   if (rc) return rc;
   if ((rc = ensure_device())) return rc;
-  const int ll = launch_layer_of(d, d->laser.pos[2]);
-  if (ll < 0) return fail(LMC_ERR_INVALID, "laser position lies outside the sample");
   lmc_scene* s = new lmc_scene();
-  s->layers.assign(d->layers, d->layers + d->n_layers);
-  s->species.assign(d->species, d->species + d->n_species);
-  if (d->n_sensors) s->sensors.assign(d->sensors, d->sensors + d->n_sensors);
-  if (d->n_tables) s->tables.assign(d->tables, d->tables + d->n_tables);
-  s->laser = d->laser;
-  s->n_medium = d->n_medium;
-  s->n_out = d->n_out;
-  s->launch_layer = ll;
-  // group sensor planes with the reference's 1e-9 key tolerance
-  // (detector.hpp:46-49), ascending z; sensors reordered by plane
-  std::vector<double> zs;
-  for (auto& sd : s->sensors) {
-    bool found = false;
-    for (double z : zs)
-      if (!(sd.z < z - 1e-9) && !(z < sd.z - 1e-9)) found = true;
-    if (!found) zs.push_back(sd.z);
-  }
-  std::sort(zs.begin(), zs.end());
-  int lds_grid = 0;
-  for (size_t pl = 0; pl < zs.size(); ++pl) {
-    PlaneDev P;
-    std::memset(&P, 0, sizeof(P));
-    P.z = zs[pl];
-    P.first = static_cast<int32_t>(s->sensors_host.size());
-    P.count = 0;
-    bool all_absorb = true;
-    for (size_t i = 0; i < s->sensors.size(); ++i) {
-      const lmc_sensor_desc& sd = s->sensors[i];
-      if ((sd.z < zs[pl] - 1e-9) || (zs[pl] < sd.z - 1e-9)) continue;
-      SensorDev o;
-      std::memset(&o, 0, sizeof(o));
-      o.kind = sd.kind;
-      o.absorb = sd.absorb;
-      o.plane = static_cast<int32_t>(pl);
-      o.id = static_cast<int32_t>(i);
-      o.f_dir_en = sd.f_dir_en;
-      o.f_dir = sd.f_dir;
-      o.f_ev_en = sd.f_ev_en;
-      o.f_ev_min = sd.f_ev_min;
-      o.f_ev_max = sd.f_ev_max;
-      o.f_th_en = sd.f_th_en;
-      o.f_ph_en = sd.f_ph_en;
-      o.f_pos_en = sd.f_pos_en;
-      o.cos_th_min = static_cast<float>(sd.cos_th_min);
-      o.cos_th_max = static_cast<float>(sd.cos_th_max);
-      o.ph_min = static_cast<float>(sd.ph_min);
-      o.ph_max = static_cast<float>(sd.ph_max);
-      o.x_min = static_cast<float>(sd.x_min);
-      o.x_max = static_cast<float>(sd.x_max);
-      o.y_min = static_cast<float>(sd.y_min);
-      o.y_max = static_cast<float>(sd.y_max);
-      for (int j = 0; j < 8; ++j) o.g[j] = sd.g[j];
-      o.n0 = sd.n[0];
-      o.n1 = sd.n[1];
-      o.n2 = sd.n[2];
-      o.out_off = sd.out_offset;
-      // privatise small angular grids per workgroup of the sensor kernel (<= 48 KiB)
-      const bool small = (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_FARFIELD_FLUENCE ||
-                          sd.kind == LMC_SENSOR_REFLECTANCE) &&
-                         (lds_grid + sd.out_len) * 8 <= 49152;
-      if (small) {
-        o.lds = 1;
-        o.lds_off = lds_grid;
-        lds_grid += static_cast<int>(sd.out_len);
-      }
-      s->sensors_host.push_back(o);
-      P.count++;
-      if (sd.kind == LMC_SENSOR_FARFIELD_CBS) P.has_cbs = 1;
-      if (!sd.absorb) all_absorb = false;
-    }
-    // a plane on or outside the stack's faces is crossed only by a photon
-    // leaving the sample; an interior plane whose sensors all absorb records
-    // at most one crossing as well
-    const double ztop_s = s->layers.back().z_max;
-    const bool boundary = zs[pl] <= 0.0 || (!std::isinf(ztop_s) && zs[pl] >= ztop_s);
-    s->plane_single.push_back(boundary || all_absorb ? 1 : 0);
-    s->planes_host.push_back(P);
-  }
-  // hit counters are indexed by position in sensors_host; keep a map back
-  s->lds_grid = lds_grid;
-  rc = s->sensors_dev.upload(s->sensors_host.data(), s->sensors_host.size());
-  if (!rc) rc = s->planes_dev.upload(s->planes_host.data(), s->planes_host.size());
-  if (!rc) rc = s->out.alloc(std::max<int64_t>(1, s->n_out));
-  if (!rc) rc = s->hits.alloc(kMaxSensors);
-  if (!rc) rc = s->counter.alloc(8);
-  if (!rc) rc = s->qcount.alloc(2);
+  rc = add_scene(s, d, true);
+  if (!rc) rc = finish_set(s);
+  if (!rc) rc = zero_set(s);
   if (rc) {
+    destroy_set(s);
     delete s;
-    return rc;
-  }
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
-      hipEventCreate(&s->ev2) != hipSuccess) {
-    delete s;
-    return fail(LMC_ERR_DEVICE, "stream/event creation failed");
-  }
-  rc = lmc_scene_zero(s);
-  if (rc) {
-    lmc_scene_destroy(s);
     return rc;
   }
   *out = s;
@@ -648,39 +926,22 @@ int lmc_scene_create(const lmc_scene_desc* d, lmc_scene** out) {
 
 void lmc_scene_destroy(lmc_scene* s) {
   if (!s) return;
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  if (s->ev0) (void)hipEventDestroy(s->ev0);
-  if (s->ev1) (void)hipEventDestroy(s->ev1);
-  if (s->ev2) (void)hipEventDestroy(s->ev2);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  destroy_set(s);
   delete s;
 }
 
 int lmc_scene_zero(lmc_scene* s) {
   if (!s) return fail(LMC_ERR_INVALID, "null scene");
-  LMC_HIP(hipMemsetAsync(s->out.p, 0, sizeof(double) * s->out.n, s->stream));
-  LMC_HIP(hipMemsetAsync(s->hits.p, 0, sizeof(unsigned long long) * s->hits.n, s->stream));
-  if (s->grad.p) LMC_HIP(hipMemsetAsync(s->grad.p, 0, sizeof(double) * s->grad.n, s->stream));
-  LMC_HIP(hipStreamSynchronize(s->stream));
-  return LMC_OK;
+  return zero_set(s);
 }
 
-int lmc_scene_run(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) { return run_dispatch<false>(s, r, st); }
+int lmc_scene_run(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) {
+  return run_dispatch<false>(s, r, nullptr, false, st);
+}
 
 int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits) {
   if (!s) return fail(LMC_ERR_INVALID, "null scene");
-  if (out) {
-    if (n != s->n_out) return fail(LMC_ERR_INVALID, "read size does not match the scene's accumulators");
-    std::vector<double> tmp(s->n_out);
-    if (s->n_out) LMC_HIP(hipMemcpy(tmp.data(), s->out.p, sizeof(double) * s->n_out, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < s->n_out; ++i) out[i] += tmp[i];
-  }
-  if (hits) {
-    unsigned long long h[kMaxSensors];
-    LMC_HIP(hipMemcpy(h, s->hits.p, sizeof(h), hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < s->sensors_host.size(); ++k) hits[s->sensors_host[k].id] += h[k];
-  }
-  return LMC_OK;
+  return read_scene(s, 0, out, n, hits);
 }
 
 int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int64_t n_grad, lmc_stats* st) {
@@ -697,10 +958,9 @@ int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int
   if (!s->grad.p || static_cast<int64_t>(s->grad.n) != need) {
     int rc = s->grad.alloc(need);
     if (rc) return rc;
-    LMC_HIP(hipMemset(s->grad.p, 0, sizeof(double) * need));
   }
   LMC_HIP(hipMemset(s->grad.p, 0, sizeof(double) * need));
-  int rc = run_dispatch<true>(s, r, st);
+  int rc = run_dispatch<true>(s, r, nullptr, false, st);
   if (rc) return rc;
   if (grad) {
     std::vector<double> tmp(need);
@@ -710,19 +970,50 @@ int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int
   return LMC_OK;
 }
 
-struct lmc_batch;
-int lmc_batch_create(int32_t, const lmc_scene_desc*, lmc_batch** out) {
-  if (out) *out = nullptr;
-  return fail(LMC_ERR_UNSUPPORTED, "batched launch not built yet");
+int lmc_batch_create(int32_t n_scenes, const lmc_scene_desc* scenes, lmc_batch** out) {
+  if (!out) return fail(LMC_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  if (n_scenes <= 0 || n_scenes > kMaxBatchScenes || !scenes)
+    return fail(LMC_ERR_INVALID, "a batch holds 1..4096 scenes");
+  int rc = ensure_device();
+  if (rc) return rc;
+  lmc_batch* b = new lmc_batch();
+  for (int k = 0; k < n_scenes && !rc; ++k) rc = add_scene(&b->set, &scenes[k], false);
+  if (!rc && b->set.sensors_host.size() > 8192) rc = fail(LMC_ERR_INVALID, "a batch holds at most 8192 sensors");
+  if (!rc) rc = finish_set(&b->set);
+  if (!rc) rc = zero_set(&b->set);
+  if (rc) {
+    destroy_set(&b->set);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return LMC_OK;
 }
-void lmc_batch_destroy(lmc_batch*) {}
-int lmc_batch_run(lmc_batch*, const uint64_t*, const lmc_run_desc*, int32_t, lmc_stats*) {
-  return fail(LMC_ERR_UNSUPPORTED, "batched launch not built yet");
+
+void lmc_batch_destroy(lmc_batch* b) {
+  if (!b) return;
+  destroy_set(&b->set);
+  delete b;
 }
-int lmc_batch_read(lmc_batch*, int32_t, double*, int64_t, uint64_t*) {
-  return fail(LMC_ERR_UNSUPPORTED, "batched launch not built yet");
+
+int lmc_batch_run(lmc_batch* b, const uint64_t* budgets, const lmc_run_desc* r, int32_t use_graph,
+                  lmc_stats* st) {
+  if (!b || !budgets || !r) return fail(LMC_ERR_INVALID, "null argument");
+  return run_dispatch<false>(&b->set, r, budgets, use_graph != 0, st);
 }
-int lmc_batch_zero(lmc_batch*) { return fail(LMC_ERR_UNSUPPORTED, "batched launch not built yet"); }
+
+int lmc_batch_read(lmc_batch* b, int32_t scene, double* out, int64_t n, uint64_t* hits) {
+  if (!b) return fail(LMC_ERR_INVALID, "null batch");
+  if (scene < 0 || scene >= static_cast<int32_t>(b->set.scenes.size()))
+    return fail(LMC_ERR_INVALID, "scene index out of range");
+  return read_scene(&b->set, scene, out, n, hits);
+}
+
+int lmc_batch_zero(lmc_batch* b) {
+  if (!b) return fail(LMC_ERR_INVALID, "null batch");
+  return zero_set(&b->set);
+}
 
 int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
   if (n <= 0 || !ctr || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");

@@ -20,7 +20,9 @@ from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFi
                       PlanarFluenceProcessed, StokesMatrixProcessed, StokesRadialProcessed,
                       postprocess_farfield_cbs, postprocess_planar_field, postprocess_planar_fluence,
                       solid_angle_bin)
-from .simulation import GpuScene, SimConfig, device_info, run_simulation_parallel, validate_sim_config
+from .simulation import (GpuBatch, GpuScene, SimConfig, device_info, run_simulation_batched, run_simulation_parallel,
+                         validate_sim_config)
+from .tuning import autotune
 
 _load_library()
 

@@ -270,6 +270,93 @@ class GpuScene:
             pass
 
 
+class GpuBatch:
+    """Many independent scenes resident on the device, run together in one
+    persistent launch (lmc_batch_*).  Scene i's photon j follows the same
+    random stream as photon j of a GpuScene run of that scene alone."""
+
+    def __init__(self, scenes):
+        """scenes: iterable of (sample, laser, sensors) triples."""
+        self.builds = [SceneBuild(sa, la, list(se)) for sa, la, se in scenes]
+        if not self.builds:
+            raise ValueError("a batch needs at least one scene")
+        arr = (_lib.SceneDesc * len(self.builds))(*[b.desc for b in self.builds])
+        self._descs = arr
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.lib().lmc_batch_create(len(self.builds), arr, ctypes.byref(self._h)), "lmc_batch_create")
+
+    def __len__(self):
+        return len(self.builds)
+
+    def run(self, budgets, run: _lib.RunDesc, use_graph: bool = False) -> dict:
+        b = np.ascontiguousarray(budgets, dtype=np.uint64)
+        if b.shape != (len(self.builds),):
+            raise ValueError("one photon budget per scene")
+        st = _lib.Stats()
+        _lib.check(_lib.lib().lmc_batch_run(self._h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            ctypes.byref(run), int(bool(use_graph)), ctypes.byref(st)),
+                   "lmc_batch_run")
+        return st.as_dict()
+
+    def read(self, k: int) -> tuple[np.ndarray, np.ndarray]:
+        bld = self.builds[k]
+        out = np.zeros(bld.n_out)
+        hits = np.zeros(max(1, len(bld.sensors)), dtype=np.uint64)
+        _lib.check(_lib.lib().lmc_batch_read(self._h, int(k), _lib.dptr(out), out.size,
+                                             hits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
+                   "lmc_batch_read")
+        return out, hits
+
+    def zero(self) -> None:
+        _lib.check(_lib.lib().lmc_batch_zero(self._h), "lmc_batch_zero")
+
+    def merge_into_sensors(self) -> None:
+        for k, bld in enumerate(self.builds):
+            out, hits = self.read(k)
+            bld.scatter_results(out, hits)
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().lmc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_simulation_batched(configs, use_graph: bool = True) -> dict:
+    """Run several SimConfigs in one batched launch and add each one's results
+    into its own sensors, as run_simulation_parallel would for each.  The
+    configs share one random key and run controls (seed, MAX_EVENTS,
+    track_reverse_paths, precision, rng, photon_offset), which must agree."""
+    configs = list(configs)
+    if not configs:
+        raise ValueError("no configs")
+    for c in configs:
+        validate_sim_config(c)
+        if c.absorption is not None:
+            raise NotImplementedError("the Absorption grid is not run by the GPU transport path")
+    keys = ("seed", "MAX_EVENTS", "track_reverse_paths", "precision", "rng", "photon_offset",
+            "block_size", "blocks_per_cu", "claim")
+    c0 = configs[0]
+    for c in configs[1:]:
+        for k in keys:
+            if getattr(c, k) != getattr(c0, k):
+                raise ValueError(f"batched configs must share {k}")
+    batch = GpuBatch([(c.sample, c.laser, c.detector.detectors) for c in configs])
+    try:
+        st = batch.run([int(c.n_photons) for c in configs], run_desc_from(c0, n_photons=0), use_graph)
+        batch.merge_into_sensors()
+    finally:
+        batch.close()
+    for c in configs:
+        c.last_stats = st
+    return st
+
+
 def run_simulation_parallel(config: SimConfig) -> None:
     """Transport config.n_photons photons on the GPU and add the results into
     config.detector's sensors (python_module.cpp:664-673 -> simulation.cpp:105-249)."""
