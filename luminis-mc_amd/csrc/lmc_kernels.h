@@ -45,14 +45,14 @@ template <class R> struct KParams {
   R* qf;
   double* qd;                   // FP64 optical path at the crossing
   int32_t* qi;
-  unsigned long long* q_count;  // [0] records written, [1] overflowed
+  unsigned long long* q_count;  // [0] slots handed out, [1] overflowed
   int64_t q_cap;
   int64_t n_out;
   uint64_t n_photons, offset;
   uint64_t local_offset;        // batched runs: first photon index within every scene
   int32_t n_layers, n_species, n_sensors, n_planes, n_scenes;
   int32_t cdf_total, guide_total, stab_total, sel_total;
-  int32_t lds_grid, max_events, claim;
+  int32_t lds_grid, max_events, claim, q_chunk;
   uint32_t key0, key1, idx_hi;
 };
 
@@ -81,6 +81,8 @@ template <class R> struct SceneView {
   const PlaneDev* planes;
   double* lgrid;     // LDS accumulators
   unsigned* lhits;   // LDS hit counters [kMaxSensors], then crossings, cbs
+  unsigned long long* wq_base;  // per wave: first slot of its current queue chunk
+  unsigned* wq_used;            // per wave: slots used in that chunk
 };
 
 // xoshiro state exists only in the sequential-stream build.
@@ -412,7 +414,11 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
   const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
   const R corr = M<R>::sqrt(dot(ov, ov));
-  atomicAdd(&v.lhits[kMaxSensors], 1u);
+  {
+    const uint64_t act = __ballot(1);
+    if (__lane_id() == __ffsll(static_cast<unsigned long long>(act)) - 1)
+      atomicAdd(&v.lhits[kMaxSensors], static_cast<unsigned>(__popcll(act)));
+  }
   const R gpl_hit = corr > R(0) ? ph.gpl - corr : ph.gpl;
   const R arr = ph.t0 + gpl_hit;
   bool killed = false;
@@ -428,15 +434,35 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
       mask |= 1 << k;
   }
   if (mask == 0) return killed;
-  // one queue reservation per wave: the recording lanes get consecutive slots,
-  // so their field-major stores coalesce
+  // Queue slots come from a per-wave chunk of q_chunk consecutive slots, so
+  // the global counter sees one atomic per chunk instead of one per crossing
+  // event; the recording lanes get consecutive slots (their field-major
+  // stores coalesce).  A wave's last chunk may stay partly unused: those
+  // slots keep the plane sentinel -1 and the sensor kernel skips them.
   const uint64_t act = __ballot(1);
   const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
   const int lane = __lane_id();
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(&p.q_count[0], static_cast<unsigned long long>(__popcll(act)));
-  base = __shfl(base, leader);
-  const unsigned long long slot = base + static_cast<unsigned long long>(__popcll(act & ((1ull << lane) - 1ull)));
+  const unsigned k = static_cast<unsigned>(__popcll(act));
+  const unsigned rank = static_cast<unsigned>(__popcll(act & ((1ull << lane) - 1ull)));
+  const int wid = static_cast<int>(threadIdx.x >> 6);
+  unsigned long long b_old = 0, b_new = 0;
+  unsigned room = 0;
+  if (lane == leader) {
+    const unsigned used = v.wq_used[wid];
+    room = static_cast<unsigned>(p.q_chunk) - used;
+    b_old = v.wq_base[wid] + used;
+    if (k <= room) {
+      v.wq_used[wid] = used + k;
+    } else {
+      b_new = atomicAdd(&p.q_count[0], static_cast<unsigned long long>(p.q_chunk));
+      v.wq_base[wid] = b_new;
+      v.wq_used[wid] = k - room;
+    }
+  }
+  b_old = __shfl(b_old, leader);
+  b_new = __shfl(b_new, leader);
+  room = __shfl(room, leader);
+  const unsigned long long slot = rank < room ? b_old + rank : b_new + (rank - room);
   if (static_cast<int64_t>(slot) >= p.q_cap) {
     atomicAdd(&p.q_count[1], 1ull);
     return killed;
@@ -891,9 +917,18 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
   };
   unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
   for (int i = tid; i < kMaxSensors + 2; i += nt) sHits[i] = 0u;
+  const int n_waves = (nt + 63) >> 6;
+  unsigned long long* sWB = reinterpret_cast<unsigned long long*>(carve(sizeof(unsigned long long) * n_waves));
+  unsigned* sWU = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * n_waves));
+  for (int i = tid; i < n_waves; i += nt) {
+    sWB[i] = 0ull;
+    sWU[i] = static_cast<unsigned>(p.q_chunk);  // full: the first crossing takes a chunk
+  }
   SceneView<R> v;
   v.lgrid = nullptr;
   v.lhits = sHits;
+  v.wq_base = sWB;
+  v.wq_used = sWU;
   uint64_t* sPre = nullptr;
   if constexpr (kBatch) {
     sPre = reinterpret_cast<uint64_t*>(carve(sizeof(uint64_t) * (p.n_scenes + 1)));
@@ -1036,6 +1071,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
     const R* f = p.qf + i;
     const int32_t* iq = p.qi + i;
     const int pl = iq[I_PLANE * c];
+    if (pl < 0) continue;  // slot of a partly used chunk
     const int mask = iq[I_MASK * c];
     const int events = iq[I_EVENTS * c];
     const R hx = f[F_HX * c], hy = f[F_HY * c];
@@ -1140,6 +1176,12 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
       if (val != 0.0) unsafeAtomicAdd(p.out + sd.out_off + i, val);
     }
   }
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = val;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {

@@ -369,6 +369,7 @@ template <class K> int raise_lds_limit(K kfn, size_t lds) {
 // paths) short against the launch; the queue takes at most a quarter of the
 // free device memory.
 constexpr int64_t kQueueRecords = int64_t(1) << 26;
+constexpr int kWaveQueueChunk = 128;  // queue slots a wave takes from the global counter at a time
 
 void drop_graph(lmc_scene* s) {
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
@@ -468,6 +469,8 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   size_t base_lds = 0;
   auto add = [&](size_t b) { base_lds += (b + 15) & ~static_cast<size_t>(15); };
   add(sizeof(unsigned) * (kMaxSensors + 2));
+  add(sizeof(unsigned long long) * 8);  // per-wave queue chunk state, up to 8 waves
+  add(sizeof(unsigned) * 8);
   if (batch) {
     add(sizeof(uint64_t) * (n_sc + 1));
   } else {
@@ -495,12 +498,16 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   const bool track = r->track_reverse_paths != 0;
   int64_t chunk = static_cast<int64_t>(total);
+  p.q_chunk = kWaveQueueChunk;
+  // slots a launch can leave unused: the rest of every wave's last chunk
+  const int64_t reserve = static_cast<int64_t>(grid) * ((block + 63) / 64) * kWaveQueueChunk;
   if (per_photon > 0) {
     const int64_t want = static_cast<int64_t>(std::min<uint64_t>(total, uint64_t(1) << 40)) * per_photon;
-    const int64_t cap = std::min<int64_t>(kQueueRecords, want);
+    const int64_t cap = std::min<int64_t>(kQueueRecords, want) + reserve;
     int rc = ensure_queue<R>(s, cap);
     if (rc) return rc;
-    chunk = std::max<int64_t>(1, s->q_cap / per_photon);
+    if (s->q_cap - reserve < per_photon) return fail(LMC_ERR_NOMEM, "crossing record queue too small");
+    chunk = std::max<int64_t>(1, (s->q_cap - reserve) / per_photon);
     p.qf = reinterpret_cast<R*>(s->queue.p);
     p.qd = reinterpret_cast<double*>(s->queue.p + sizeof(R) * kRecF * s->q_cap);
     p.qi = reinterpret_cast<int32_t*>(s->queue.p + (sizeof(R) * kRecF + sizeof(double)) * s->q_cap);
@@ -544,6 +551,11 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->counter.p, 1);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->qcount.p, 1);
+    if (p.q_cap > 0) {  // plane sentinel over the slots this launch can hand out
+      const int64_t n_fill = std::min<int64_t>(p.q_cap, static_cast<int64_t>(L.n) * per_photon + reserve);
+      hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(std::min<int64_t>((n_fill + 255) / 256, 4096))),
+                         dim3(256), 0, s->stream, p.qi + I_PLANE * p.q_cap, n_fill, -1);
+    }
     LMC_HIP(hipGetLastError());
     return LMC_OK;
   };
