@@ -99,6 +99,82 @@ def plain_mie_rate(n: int):
     return n / dt, st
 
 
+def batched_leg(frac_plain: float = 0.25):
+    """The 256-scene batched workload in one persistent launch (hipGraph
+    replay, autotuned launch shape) against the same scenes run one after
+    another as single-scene launches."""
+    import numpy as np
+    import batch_scenes
+    from luminis_mc.simulation import GpuBatch, GpuScene, SimConfig, run_desc_from
+    from luminis_mc.tuning import autotune
+    specs = batch_scenes.load()
+    built = [batch_scenes.build_scene(s) for s in specs]
+    budgets = np.array([s["budget"] for s in specs], dtype=np.uint64)
+    c = SimConfig()
+    c.seed, c.track_reverse_paths = 4242, True
+    b = GpuBatch([(sa, la, g.detectors) for sa, la, g, _, _ in built])
+    tune = autotune(b, run_desc_from(c, n_photons=0), budgets=budgets, fraction=0.05)
+    c.block_size, c.blocks_per_cu, c.claim = tune["block_size"], tune["blocks_per_cu"], tune["claim"]
+    r = run_desc_from(c, n_photons=0)
+    b.run(budgets, r, use_graph=True)  # capture + first replay
+    b.zero()
+    t = time.perf_counter()
+    st = b.run(budgets, r, use_graph=True)
+    dt_full = time.perf_counter() - t
+    # same scenes at a fraction of the budgets: batched vs one launch per scene
+    small = np.maximum(1, np.round(budgets * frac_plain)).astype(np.uint64)
+    b.run(small, r)
+    t = time.perf_counter()
+    st_small = b.run(small, r)
+    dt_b = time.perf_counter() - t
+    b.close()
+    c2 = SimConfig()
+    c2.seed, c2.track_reverse_paths = 4242, True
+    scs = []
+    for k, (sa, la, g, _, _) in enumerate(built):
+        sc = GpuScene(sa, la, g.detectors)
+        sc.run(run_desc_from(c2, n_photons=int(small[k])))
+        scs.append(sc)
+    t = time.perf_counter()
+    for k, sc in enumerate(scs):
+        sc.run(run_desc_from(c2, n_photons=int(small[k])))
+    dt_p = time.perf_counter() - t
+    for sc in scs:
+        sc.close()
+    return {"workload": "256 independent scenes, 1-4 layers, HG/Mie, CBS 100x1 each (tests/data/batch256.json)",
+            "photons": int(budgets.sum()), "photons_per_s": round(float(budgets.sum()) / dt_full, 1),
+            "events_per_photon": round(st["events"] / max(1, st["photons"]), 2), "graph": True,
+            "launch_shape": {k: tune[k] for k in ("block_size", "blocks_per_cu", "claim")},
+            "vs_plain": {"budget_fraction": frac_plain, "photons": int(small.sum()),
+                         "batched_photons_per_s": round(float(small.sum()) / dt_b, 1),
+                         "plain_photons_per_s": round(float(small.sum()) / dt_p, 1),
+                         "speedup": round(dt_p / dt_b, 2)}}
+
+
+def adjoint_leg(n: int = 20_000_000):
+    """Forward-only vs forward + score gradients (5 layer parameters) on the
+    two-media gradient workload."""
+    import scenes
+    import luminis_mc as lmc
+    sample, laser, sensors, keep = scenes.scene_adjoint()
+    tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=n, seed=99)
+    th = tr.theta0()
+    sc = tr.scene(th)
+    tr.evaluate(th, with_grad=False, n_photons=min(n, 10 ** 6), scene=sc)
+    tr.evaluate(th, with_grad=True, n_photons=min(n, 10 ** 6), scene=sc)
+    t = time.perf_counter()
+    tr.evaluate(th, with_grad=False, scene=sc)
+    dt_f = time.perf_counter() - t
+    t = time.perf_counter()
+    vals, jac = tr.evaluate(th, with_grad=True, scene=sc)
+    dt_a = time.perf_counter() - t
+    sc.close()
+    return {"workload": "Mie r=0.3um [0,1) mm over HG g=0.8; reflectance + CBS 100x1; d/d(mu_s, mu_a) both "
+                        "layers, d/dg HG layer", "photons": n,
+            "forward_photons_per_s": round(n / dt_f, 1), "adjoint_photons_per_s": round(n / dt_a, 1),
+            "reflectance": float(vals[0]), "dR_dtheta": [[float(x) for x in row] for row in jac[:, :, 0]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +184,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -170,6 +248,10 @@ def main():
         out["plain_mie_cbs"] = {"photons_per_s": round(rate, 1), "photons": 10 ** 7,
                                 "events_per_photon": round(pst["events"] / max(1, pst["photons"]), 2),
                                 "kernel_ms": round(pst["kernel_ms"], 3)}
+    if not args.no_batch:
+        out["batched"] = batched_leg()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         cb = cpu_baseline(scene.build, args.cpu_seconds)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}

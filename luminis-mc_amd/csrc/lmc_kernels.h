@@ -53,6 +53,7 @@ template <class R> struct KParams {
   int32_t n_layers, n_species, n_sensors, n_planes, n_scenes;
   int32_t cdf_total, guide_total, stab_total, sel_total;
   int32_t lds_grid, max_events, claim, q_chunk;
+  int32_t lds_grad;             // adjoint runs: privatised grids keep their gradient rows in LDS too
   uint32_t key0, key1, idx_hi;
 };
 
@@ -303,9 +304,15 @@ __device__ __forceinline__ void deposit(const KParams<R>& p, const SensorDev& sd
                                         double val, const R* score) {
   acc_add(lds_grid + sd.lds_off, p.out + sd.out_off, sd.lds != 0, off, val);
   if constexpr (kAdj) {
+    // gradient row q of this value: LDS copy after the value grids when the
+    // grid is privatised (a one-cell grid would otherwise take every
+    // photon's atomics on one address), HBM otherwise
     const int np = 3 * (p.n_layers < kMaxGradLayers ? p.n_layers : kMaxGradLayers);
+    const bool lg = sd.lds != 0 && p.lds_grad != 0;
     for (int q = 0; q < np; ++q)
-      if (score[q] != R(0)) unsafeAtomicAdd(p.grad + q * p.n_out + sd.out_off + off, val * static_cast<double>(score[q]));
+      if (score[q] != R(0))
+        acc_add(lds_grid + static_cast<int64_t>(p.lds_grid) * (1 + q) + sd.lds_off, p.grad + q * p.n_out + sd.out_off,
+                lg, off, val * static_cast<double>(score[q]));
   }
 }
 
@@ -1059,9 +1066,11 @@ template <class R, bool kAdj>
 __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* sgrid = reinterpret_cast<double*>(smem);
-  unsigned* shits = reinterpret_cast<unsigned*>(smem + ((sizeof(double) * p.lds_grid + 15) & ~static_cast<size_t>(15)));
+  const int n_gp = 3 * (p.n_layers < kMaxGradLayers ? p.n_layers : kMaxGradLayers);
+  const int64_t lds_vals = static_cast<int64_t>(p.lds_grid) * (1 + ((kAdj && p.lds_grad) ? n_gp : 0));
+  unsigned* shits = reinterpret_cast<unsigned*>(smem + ((sizeof(double) * lds_vals + 15) & ~static_cast<size_t>(15)));
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < p.lds_grid; i += nt) sgrid[i] = 0.0;
+  for (int64_t i = tid; i < lds_vals; i += nt) sgrid[i] = 0.0;
   for (int i = tid; i < p.n_sensors + 1; i += nt) shits[i] = 0u;
   __syncthreads();
   unsigned* s_cbs = shits + p.n_sensors;
@@ -1174,6 +1183,13 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
     for (int64_t i = tid; i < len; i += nt) {
       const double val = sgrid[sd.lds_off + i];
       if (val != 0.0) unsafeAtomicAdd(p.out + sd.out_off + i, val);
+    }
+    if (kAdj && p.lds_grad) {
+      for (int q = 0; q < n_gp; ++q)
+        for (int64_t i = tid; i < len; i += nt) {
+          const double val = sgrid[static_cast<int64_t>(p.lds_grid) * (1 + q) + sd.lds_off + i];
+          if (val != 0.0) unsafeAtomicAdd(p.grad + q * p.n_out + sd.out_off + i, val);
+        }
     }
   }
 }

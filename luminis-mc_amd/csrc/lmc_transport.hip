@@ -524,7 +524,10 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   int rc = raise_lds_limit(kt, lds);
   if (rc) return rc;
   void (*kr)(KParams<R>) = k_crossings<R, kAdj>;
-  const size_t lds_r = ((sizeof(double) * p.lds_grid + 15) & ~15ull) + sizeof(unsigned) * (p.n_sensors + 1);
+  const int n_gp = 3 * std::min(p.n_layers, kMaxGradLayers);
+  p.lds_grad = (kAdj && sizeof(double) * p.lds_grid * (1 + n_gp) <= 131072) ? 1 : 0;
+  const size_t lds_vals = static_cast<size_t>(p.lds_grid) * (1 + (p.lds_grad ? n_gp : 0));
+  const size_t lds_r = ((sizeof(double) * lds_vals + 15) & ~15ull) + sizeof(unsigned) * (p.n_sensors + 1);
   if (lds_r > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "too many sensors for the sensor kernel");
   if ((rc = raise_lds_limit(kr, lds_r))) return rc;
   // the launch sequence: [zero work counter + record count, transport, sensors] per launch

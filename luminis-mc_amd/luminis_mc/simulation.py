@@ -224,11 +224,16 @@ def run_desc_from(config: SimConfig, n_photons: int | None = None, photon_offset
 class GpuScene:
     """A scene resident on the device: create once, run many times."""
 
-    def __init__(self, sample: Sample, laser: Laser, sensors):
-        self.build = SceneBuild(sample, laser, list(sensors))
+    def __init__(self, sample: Sample, laser: Laser, sensors, build: SceneBuild | None = None):
+        self.build = build if build is not None else SceneBuild(sample, laser, list(sensors))
         self._h = ctypes.c_void_p()
         _lib.check(_lib.lib().lmc_scene_create(ctypes.byref(self.build.desc), ctypes.byref(self._h)),
                    "lmc_scene_create")
+
+    @classmethod
+    def from_build(cls, build) -> "GpuScene":
+        """A device scene from an already flattened descriptor (SceneBuild or alike)."""
+        return cls(None, None, None, build=build)
 
     def run(self, run: _lib.RunDesc) -> dict:
         st = _lib.Stats()

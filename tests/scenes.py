@@ -92,3 +92,26 @@ def config_multilayer(n_photons: int, seed: int = 0, small: bool = True, with_fi
     cfg.sample, cfg.laser, cfg.detector = sample, laser, group
     cfg._keep = (m1, m2, m3, ph1, ph2, ph3)
     return cfg, {"cbs": cbs, "fluence": flu, "ff_fluence": fff, "reflectance": refl, "field": field}
+
+
+def scene_adjoint(n_theta: int = 100):
+    """Two media for the gradient workload: [0, 1) mm Mie r = 0.3 um (n_p = 1.59 in
+    n = 1.33; mu_s 15, mu_a 0.01 /mm) over a semi-infinite Henyey-Greenstein
+    g = 0.8 layer on an RGD r = 0.1 um amplitude host (mu_s 8, mu_a 0.05 /mm);
+    linearly polarised point beam; total reflectance and far-field CBS 100 theta x
+    1 phi over 0..0.1 rad at z = 0.  Returns (sample, laser, sensors, keep)."""
+    n = 1.33
+    m1, ph1 = mie_medium(0.3e-3, 1.59, n, 15.0, 0.01)
+    ph2 = lmc.HenyeyGreensteinPhaseFunction(0.8)
+    m2 = lmc.RGDMedium(ph2, 0.1e-3, 1.59, n, WAVELENGTH)
+    m2.set_scattering_coefficient(8.0)
+    m2.set_absorption_coefficient(0.05)
+    m2.set_mean_free_path(1.0 / 8.05)
+    sample = lmc.Sample(n)
+    sample.add_layer(m1, 0.0, 1.0)
+    sample.add_layer(m2, 1.0, math.inf)
+    laser = lmc.Laser(1.0, 0.0, WAVELENGTH, 0.0, lmc.LaserSource.Point)
+    group = lmc.SensorsGroup()
+    refl = group.add_detector(lmc.ReflectanceSensor(0.0, absorb=False))
+    cbs = group.add_detector(cbs_sensor(n_theta, 0.1, 1))
+    return sample, laser, group.detectors, (m1, ph1, m2, ph2, refl, cbs)

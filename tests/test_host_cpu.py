@@ -169,3 +169,39 @@ def test_oracle_runs_plain_mie_and_is_thread_count_invariant():
     assert np.all(g[0] >= 0) and np.all(g[4] >= 0)
     # single scatter photons deposit coh == incoh; multiple ones bound coh <= 2 incoh
     assert np.all(g[0] <= 2 * g[4] + 1e-12)
+
+
+def test_autograd_function_chain_rule_without_device(monkeypatch):
+    """The autograd wrapper returns J^T g (CPU check with a stand-in evaluate)."""
+    import torch
+    import scenes
+    sample, laser, sensors, keep = scenes.scene_adjoint()
+    tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=10)
+    rng = np.random.default_rng(3)
+    vals = rng.normal(size=tr.n_out)
+    jac = rng.normal(size=(tr.n_layers, 3, tr.n_out))
+    monkeypatch.setattr(tr, "evaluate", lambda th, with_grad=True, **kw: (vals, jac if with_grad else None))
+    theta = torch.tensor(tr.theta0(), requires_grad=True)
+    out = tr(theta)
+    g = torch.tensor(rng.normal(size=tr.n_out))
+    (out * g).sum().backward()
+    assert np.allclose(theta.grad.numpy(), np.einsum("lpd,d->lp", jac, g.numpy()))
+    assert np.allclose(out.detach().numpy(), vals)
+
+
+def test_param_build_rewrites_coefficients():
+    import scenes
+    from luminis_mc.adjoint import _ParamBuild
+    sample, laser, sensors, keep = scenes.scene_adjoint()
+    tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=10)
+    th = tr.theta0()
+    assert np.allclose(th, [[15.0, 0.01, 0.0], [8.0, 0.05, 0.8]])
+    th2 = th * np.array([[1.1, 2.0, 1.0], [0.9, 0.5, 1.05]])
+    b = _ParamBuild(tr.base, th2)
+    for l in range(2):
+        assert b.layers[l].mu_s == th2[l, 0] and b.layers[l].mu_a == th2[l, 1]
+        assert b.layers[l].mfp == pytest.approx(1.0 / (th2[l, 0] + th2[l, 1]))
+    assert b.species[1].hg_g == pytest.approx(0.84)
+    assert tr.base.layers[0].mu_s == 15.0  # the template is untouched
+    with pytest.raises(ValueError):
+        _ParamBuild(tr.base, np.array([[15.0, 0.01, 0.0], [8.0, 0.05, 1.5]]))
