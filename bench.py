@@ -175,6 +175,26 @@ def adjoint_leg(n: int = 20_000_000):
             "reflectance": float(vals[0]), "dR_dtheta": [[float(x) for x in row] for row in jac[:, :, 0]]}
 
 
+def polydisperse_leg(n: int = 200_000_000):
+    """The polydisperse Mie mixture (32 lognormal radius bins, 2048-node S tables)
+    with far-field CBS 512 x 128 and a 1024 x 1024 planar field (FP64 phase)."""
+    import scenes
+    from luminis_mc.simulation import GpuScene, SimConfig, run_desc_from
+    sample, laser, sensors, keep = scenes.scene_polydisperse()
+    sc = GpuScene(sample, laser, sensors)
+    c = SimConfig()
+    c.seed, c.track_reverse_paths = 5150, True
+    sc.run(run_desc_from(c, n_photons=min(n, 10 ** 7)))
+    t = time.perf_counter()
+    st = sc.run(run_desc_from(c, n_photons=n, photon_offset=10 ** 7))
+    dt = time.perf_counter() - t
+    sc.close()
+    return {"workload": "semi-infinite mixture of 32 Mie radius bins (lognormal, median 0.4 um, sigma_ln 0.3), "
+                        "mu_s 30, mu_a 0.002 /mm; CBS 512x128 + planar field 1024x1024", "photons": n,
+            "photons_per_s": round(n / dt, 1), "events_per_photon": round(st["events"] / max(1, st["photons"]), 2),
+            "sensor_kernel_ms": round(st["sensor_ms"], 3), "transport_ms": round(st["transport_ms"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +206,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-poly", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -252,6 +273,8 @@ def main():
         out["batched"] = batched_leg()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
+    if not args.no_poly:
+        out["polydisperse"] = polydisperse_leg()
     if not args.no_cpu:
         cb = cpu_baseline(scene.build, args.cpu_seconds)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}

@@ -115,3 +115,39 @@ def scene_adjoint(n_theta: int = 100):
     refl = group.add_detector(lmc.ReflectanceSensor(0.0, absorb=False))
     cbs = group.add_detector(cbs_sensor(n_theta, 0.1, 1))
     return sample, laser, group.detectors, (m1, ph1, m2, ph2, refl, cbs)
+
+
+def scene_polydisperse(K: int = 32, n_table: int = 2048, n_div: int = 10000, n_theta: int = 512, n_phi: int = 128,
+                       field_n: int = 1024, field_len: float = 4.0, theta_max: float = 0.1):
+    """Polydisperse Mie suspension: a semi-infinite mixture of K radius bins of a
+    lognormal distribution (median 0.4 um, sigma_ln 0.3, +-3 sigma, equal widths
+    in ln r; bin weight = number fraction), n_p = 1.59 in n = 1.33, each species a
+    MieMedium with n_table-node S1/S2 tables; number densities scaled to mu_s =
+    30 /mm in total, mu_a = 0.002 /mm; linearly polarised point beam; far-field
+    CBS n_theta x n_phi over 0..theta_max and a PlanarFieldSensor field_n x field_n
+    over field_len x field_len mm at z = 0.  Returns (sample, laser, sensors, keep)."""
+    n = 1.33
+    mu_s_total, mu_a_total = 30.0, 0.002
+    ln_med, s_ln = math.log(0.4e-3), 0.3
+    edges = [ln_med - 3 * s_ln + 6 * s_ln * k / K for k in range(K + 1)]
+    phi = [0.5 * (1.0 + math.erf((e - ln_med) / (s_ln * math.sqrt(2.0)))) for e in edges]
+    species, weights, keep = [], [], []
+    for k in range(K):
+        r = math.exp(0.5 * (edges[k] + edges[k + 1]))
+        m, ph = mie_medium(r, 1.59, n, mu_s_total, mu_a_total, n_div=n_div, n_table=n_table)
+        species.append(m)
+        weights.append(phi[k + 1] - phi[k])
+        keep += [m, ph]
+    scale = mu_s_total / sum(w * m.scattering_cross_section() for w, m in zip(weights, species))
+    dens = [w * scale for w in weights]
+    sample = lmc.Sample(n)
+    sample.add_mixture_layer(species, dens, 0.0, math.inf)
+    layer = sample.layers[0]
+    laser = lmc.Laser(1.0, 0.0, WAVELENGTH, 0.0, lmc.LaserSource.Point)
+    group = lmc.SensorsGroup()
+    cbs = group.add_detector(lmc.FarFieldCBSSensor(theta_max, 2 * math.pi, 0.0, theta_max / n_theta,
+                                                   2 * math.pi / n_phi, 0.0))
+    field = group.add_detector(lmc.PlanarFieldSensor(0.0, field_len, field_len, field_len / field_n,
+                                                     field_len / field_n, absorb=False))
+    keep += [layer, cbs, field]
+    return sample, laser, group.detectors, keep
