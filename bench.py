@@ -75,6 +75,32 @@ def cpu_baseline(build, seconds: float):
                       f"(xoshiro per-photon streams, std::thread clone/merge), {dt:.1f} s wall"}
 
 
+def cpu_rate_scenes(builds, budgets, seconds: float, seed: int = 777):
+    """Oracle photons/s over several scenes, each run with photons in
+    proportion to its budget (one std::thread pool per scene, all host cores),
+    sized to about `seconds` of CPU work by a short calibration pass."""
+    import numpy as np
+    import oracle
+    threads = os.cpu_count() or 1
+    w = np.asarray(budgets, dtype=np.float64)
+    w = w / w.sum()
+
+    def run(total):
+        t = time.perf_counter()
+        n_tot = 0
+        for b, f in zip(builds, w):
+            n = max(threads, int(round(total * f)))
+            oracle.run(b, n, seed, n_threads=threads)
+            n_tot += n
+        return n_tot, time.perf_counter() - t
+
+    n0, t0 = run(20000 * len(builds))
+    target = n0 / t0 * seconds
+    n, dt = run(target)
+    return {"value": round(n / dt, 1), "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"{n} photons over {len(builds)} scenes in proportion to their budgets, {dt:.1f} s wall"}
+
+
 def copy_probe(mib: float):
     """One device-to-device copy of a known size, for scaling FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -99,7 +125,7 @@ def plain_mie_rate(n: int):
     return n / dt, st
 
 
-def batched_leg(frac_plain: float = 0.25):
+def batched_leg(frac_plain: float = 0.25, cpu_seconds: float = 0.0):
     """The 256-scene batched workload in one persistent launch (hipGraph
     replay, autotuned launch shape) against the same scenes run one after
     another as single-scene launches."""
@@ -141,7 +167,11 @@ def batched_leg(frac_plain: float = 0.25):
     dt_p = time.perf_counter() - t
     for sc in scs:
         sc.close()
-    return {"workload": "256 independent scenes, 1-4 layers, HG/Mie, CBS 100x1 each (tests/data/batch256.json)",
+    cpu = None
+    if cpu_seconds > 0:  # every 16th scene
+        idx = list(range(0, len(built), 16))
+        cpu = cpu_rate_scenes([_build_of(built[k]) for k in idx], [budgets[k] for k in idx], cpu_seconds)
+    res = {"workload": "256 independent scenes, 1-4 layers, HG/Mie, CBS 100x1 each (tests/data/batch256.json)",
             "photons": int(budgets.sum()), "photons_per_s": round(float(budgets.sum()) / dt_full, 1),
             "events_per_photon": round(st["events"] / max(1, st["photons"]), 2), "graph": True,
             "launch_shape": {k: tune[k] for k in ("block_size", "blocks_per_cu", "claim")},
@@ -149,6 +179,16 @@ def batched_leg(frac_plain: float = 0.25):
                          "batched_photons_per_s": round(float(small.sum()) / dt_b, 1),
                          "plain_photons_per_s": round(float(small.sum()) / dt_p, 1),
                          "speedup": round(dt_p / dt_b, 2)}}
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
+        res["speedup_vs_cpu"] = round(res["photons_per_s"] / cpu["value"], 2)
+    return res
+
+
+def _build_of(scene_tuple):
+    from luminis_mc.simulation import SceneBuild
+    sa, la, g, _, _ = scene_tuple
+    return SceneBuild(sa, la, g.detectors)
 
 
 def adjoint_leg(n: int = 20_000_000):
@@ -175,7 +215,7 @@ def adjoint_leg(n: int = 20_000_000):
             "reflectance": float(vals[0]), "dR_dtheta": [[float(x) for x in row] for row in jac[:, :, 0]]}
 
 
-def polydisperse_leg(n: int = 200_000_000):
+def polydisperse_leg(n: int = 200_000_000, cpu_seconds: float = 0.0):
     """The polydisperse Mie mixture (32 lognormal radius bins, 2048-node S tables)
     with far-field CBS 512 x 128 and a 1024 x 1024 planar field (FP64 phase)."""
     import scenes
@@ -188,11 +228,16 @@ def polydisperse_leg(n: int = 200_000_000):
     t = time.perf_counter()
     st = sc.run(run_desc_from(c, n_photons=n, photon_offset=10 ** 7))
     dt = time.perf_counter() - t
+    build = sc.build
     sc.close()
-    return {"workload": "semi-infinite mixture of 32 Mie radius bins (lognormal, median 0.4 um, sigma_ln 0.3), "
+    res = {"workload": "semi-infinite mixture of 32 Mie radius bins (lognormal, median 0.4 um, sigma_ln 0.3), "
                         "mu_s 30, mu_a 0.002 /mm; CBS 512x128 + planar field 1024x1024", "photons": n,
             "photons_per_s": round(n / dt, 1), "events_per_photon": round(st["events"] / max(1, st["photons"]), 2),
             "sensor_kernel_ms": round(st["sensor_ms"], 3), "transport_ms": round(st["transport_ms"], 3)}
+    if cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_rate_scenes([build], [1], cpu_seconds)
+        res["speedup_vs_cpu"] = round(res["photons_per_s"] / res["cpu_baseline"]["value"], 2)
+    return res
 
 
 def main():
@@ -270,11 +315,11 @@ def main():
                                 "events_per_photon": round(pst["events"] / max(1, pst["photons"]), 2),
                                 "kernel_ms": round(pst["kernel_ms"], 3)}
     if not args.no_batch:
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(cpu_seconds=0.0 if args.no_cpu else 10.0)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_poly:
-        out["polydisperse"] = polydisperse_leg()
+        out["polydisperse"] = polydisperse_leg(cpu_seconds=0.0 if args.no_cpu else 8.0)
     if not args.no_cpu:
         cb = cpu_baseline(scene.build, args.cpu_seconds)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}

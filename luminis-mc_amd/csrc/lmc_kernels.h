@@ -60,14 +60,39 @@ template <class R> struct KParams {
 // One queued plane crossing accepted by at least one sensor of its plane:
 // everything the sensors' process_hit needs (detector.cpp:178-270 builds the
 // same InteractionInfo), so the transport kernel carries none of the sensor
-// code.  The CBS history fields (frames keep their m and s rows) are written
-// only for photons with >= 2 events at a plane holding a CBS sensor, the
-// scores only in adjoint runs.  Structure of arrays, field-major, stride q_cap.
-enum RecF {
-  F_HX = 0, F_HY = 1, F_ARR = 2, F_W = 3, F_EM = 4, F_P = 8,  // light part
-  F_M1 = 17, F_S1 = 20, F_MQ = 23, F_SQ = 26, F_T = 29, F_D = 37,  // CBS history
-  F_SCORE = 40, kRecF = 52
-};
+// code.  Array of structures: each record is kRecQ quads of R written with
+// 16-byte vector stores, so a record fills whole cache lines at once.
+//   Q0 hit x, hit y, arrival path, weight      Q1 E_m, E_n (re, im)
+//   Q2-Q4 exit frame rows m, n, s (+ first-frame m)
+//   Q4-Q9 CBS history: frame after the first scatter (m, s rows), frame
+//         before the last one (m, s), raw interior product T (4 complex),
+//         r_n - r_1                             (photons with >= 2 events at
+//                                               a plane holding a CBS sensor)
+//   Q10-Q12 adjoint scores (adjoint runs only)
+// The plane, sensor mask, event count and species pair go to int arrays, the
+// FP64 optical path to a double array (both indexed by slot).
+constexpr int kRecQ = 13;
+constexpr int kRecQLight = 5, kRecQCbs = 10;
+
+template <class R> __device__ __forceinline__ void put4(R* f, int q, R a, R b, R c, R d) {
+  if constexpr (sizeof(R) == 4) {
+    reinterpret_cast<float4*>(f)[q] = make_float4(a, b, c, d);
+  } else {
+    reinterpret_cast<double2*>(f)[2 * q] = make_double2(a, b);
+    reinterpret_cast<double2*>(f)[2 * q + 1] = make_double2(c, d);
+  }
+}
+template <class R> struct Quad { R a, b, c, d; };
+template <class R> __device__ __forceinline__ Quad<R> get4(const R* f, int q) {
+  if constexpr (sizeof(R) == 4) {
+    const float4 v = reinterpret_cast<const float4*>(f)[q];
+    return {v.x, v.y, v.z, v.w};
+  } else {
+    const double2 u = reinterpret_cast<const double2*>(f)[2 * q];
+    const double2 v = reinterpret_cast<const double2*>(f)[2 * q + 1];
+    return {u.x, u.y, v.x, v.y};
+  }
+}
 enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, kRecI = 4 };
 
 template <class R> struct SceneView {
@@ -443,8 +468,8 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   if (mask == 0) return killed;
   // Queue slots come from a per-wave chunk of q_chunk consecutive slots, so
   // the global counter sees one atomic per chunk instead of one per crossing
-  // event; the recording lanes get consecutive slots (their field-major
-  // stores coalesce).  A wave's last chunk may stay partly unused: those
+  // event; the recording lanes get consecutive slots, so their records fill
+  // neighbouring cache lines.  A wave's last chunk may stay partly unused: those
   // slots keep the plane sentinel -1 and the sensor kernel skips them.
   const uint64_t act = __ballot(1);
   const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
@@ -475,36 +500,27 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
     return killed;
   }
   const int64_t c = p.q_cap;
-  R* f = p.qf + slot;
+  R* f = p.qf + static_cast<int64_t>(slot) * (4 * kRecQ);
   double opl_hit = ph.opl;
   if (corr > R(0)) opl_hit -= static_cast<double>(corr) * static_cast<double>(v.L[ph.layer].n);
-  f[F_HX * c] = hp.x;
-  f[F_HY * c] = hp.y;
-  f[F_ARR * c] = arr;
-  f[F_W * c] = ph.w;
-  f[(F_EM + 0) * c] = ph.Em.r;
-  f[(F_EM + 1) * c] = ph.Em.i;
-  f[(F_EM + 2) * c] = ph.En.r;
-  f[(F_EM + 3) * c] = ph.En.i;
-  {
-    const R fr[9] = {ph.P.m.x, ph.P.m.y, ph.P.m.z, ph.P.n.x, ph.P.n.y, ph.P.n.z, ph.P.s.x, ph.P.s.y, ph.P.s.z};
-#pragma unroll
-    for (int k = 0; k < 9; ++k) f[(F_P + k) * c] = fr[k];
-  }
+  put4(f, 0, hp.x, hp.y, arr, ph.w);
+  put4(f, 1, ph.Em.r, ph.Em.i, ph.En.r, ph.En.i);
+  put4(f, 2, ph.P.m.x, ph.P.m.y, ph.P.m.z, ph.P.n.x);
+  put4(f, 3, ph.P.n.y, ph.P.n.z, ph.P.s.x, ph.P.s.y);
+  put4(f, 4, ph.P.s.z, ph.m1.x, ph.m1.y, ph.m1.z);
   p.qd[slot] = opl_hit;
   if (ph.events >= 2 && v.planes[pl].has_cbs) {
-    const R h[23] = {ph.m1.x, ph.m1.y, ph.m1.z, ph.s1.x, ph.s1.y, ph.s1.z, ph.mq.x, ph.mq.y,
-                     ph.mq.z, ph.sq.x, ph.sq.y, ph.sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r, ph.T[1].i,
-                     ph.T[2].r, ph.T[2].i, ph.T[3].r, ph.T[3].i, ph.rn.x - ph.r1.x, ph.rn.y - ph.r1.y,
-                     ph.rn.z - ph.r1.z};
-#pragma unroll
-    for (int k = 0; k < 23; ++k) f[(F_M1 + k) * c] = h[k];
+    put4(f, 5, ph.s1.x, ph.s1.y, ph.s1.z, ph.mq.x);
+    put4(f, 6, ph.mq.y, ph.mq.z, ph.sq.x, ph.sq.y);
+    put4(f, 7, ph.sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r);
+    put4(f, 8, ph.T[1].i, ph.T[2].r, ph.T[2].i, ph.T[3].r);
+    put4(f, 9, ph.T[3].i, ph.rn.x - ph.r1.x, ph.rn.y - ph.r1.y, ph.rn.z - ph.r1.z);
   }
   if constexpr (kAdj) {
     R score[3 * kMaxGradLayers];
     adj_scores(p, v, ph, corr, score);
 #pragma unroll
-    for (int q = 0; q < 3 * kMaxGradLayers; ++q) f[(F_SCORE + q) * c] = score[q];
+    for (int q = 0; q < 3; ++q) put4(f, 10 + q, score[4 * q], score[4 * q + 1], score[4 * q + 2], score[4 * q + 3]);
   }
   int32_t* iq = p.qi + slot;
   iq[I_PLANE * c] = pl;
@@ -1077,24 +1093,29 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   const int64_t n_rec = static_cast<int64_t>(p.q_count[0]) < p.q_cap ? static_cast<int64_t>(p.q_count[0]) : p.q_cap;
   const int64_t c = p.q_cap;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * nt + tid; i < n_rec; i += static_cast<int64_t>(gridDim.x) * nt) {
-    const R* f = p.qf + i;
+    const R* f = p.qf + i * (4 * kRecQ);
     const int32_t* iq = p.qi + i;
     const int pl = iq[I_PLANE * c];
     if (pl < 0) continue;  // slot of a partly used chunk
     const int mask = iq[I_MASK * c];
     const int events = iq[I_EVENTS * c];
-    const R hx = f[F_HX * c], hy = f[F_HY * c];
-    const R arr = f[F_ARR * c];
-    const R w = f[F_W * c];
-    const cx<R> Em{f[(F_EM + 0) * c], f[(F_EM + 1) * c]}, En{f[(F_EM + 2) * c], f[(F_EM + 3) * c]};
+    const Quad<R> q0 = get4(f, 0), q1 = get4(f, 1), q2 = get4(f, 2), q3 = get4(f, 3), q4 = get4(f, 4);
+    const R hx = q0.a, hy = q0.b, arr = q0.c, w = q0.d;
+    const cx<R> Em{q1.a, q1.b}, En{q1.c, q1.d};
     Frame<R> P;
-    P.m = {f[(F_P + 0) * c], f[(F_P + 1) * c], f[(F_P + 2) * c]};
-    P.n = {f[(F_P + 3) * c], f[(F_P + 4) * c], f[(F_P + 5) * c]};
-    P.s = {f[(F_P + 6) * c], f[(F_P + 7) * c], f[(F_P + 8) * c]};
+    P.m = {q2.a, q2.b, q2.c};
+    P.n = {q2.d, q3.a, q3.b};
+    P.s = {q3.c, q3.d, q4.a};
     R score[3 * kMaxGradLayers];
     if constexpr (kAdj) {
 #pragma unroll
-      for (int qq = 0; qq < 3 * kMaxGradLayers; ++qq) score[qq] = f[(F_SCORE + qq) * c];
+      for (int qq = 0; qq < 3; ++qq) {
+        const Quad<R> sq = get4(f, 10 + qq);
+        score[4 * qq] = sq.a;
+        score[4 * qq + 1] = sq.b;
+        score[4 * qq + 2] = sq.c;
+        score[4 * qq + 3] = sq.d;
+      }
     }
     const LaunchConst<R>& lc = p.lcs[p.planes[pl].scene];
     const R ws = M<R>::sqrt(w);
@@ -1114,14 +1135,11 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
         if (events >= 2) {
           const int spk = iq[I_SP * c];
-          const Frame<R> P1 = frame_of(V3<R>{f[(F_M1 + 0) * c], f[(F_M1 + 1) * c], f[(F_M1 + 2) * c]},
-                                       V3<R>{f[(F_S1 + 0) * c], f[(F_S1 + 1) * c], f[(F_S1 + 2) * c]});
-          const Frame<R> Pq = frame_of(V3<R>{f[(F_MQ + 0) * c], f[(F_MQ + 1) * c], f[(F_MQ + 2) * c]},
-                                       V3<R>{f[(F_SQ + 0) * c], f[(F_SQ + 1) * c], f[(F_SQ + 2) * c]});
-          cx<R> T[4];
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) T[kk] = {f[(F_T + 2 * kk) * c], f[(F_T + 2 * kk + 1) * c]};
-          const V3<R> d{f[(F_D + 0) * c], f[(F_D + 1) * c], f[(F_D + 2) * c]};
+          const Quad<R> q5 = get4(f, 5), q6 = get4(f, 6), q7 = get4(f, 7), q8 = get4(f, 8), q9 = get4(f, 9);
+          const Frame<R> P1 = frame_of(V3<R>{q4.b, q4.c, q4.d}, V3<R>{q5.a, q5.b, q5.c});
+          const Frame<R> Pq = frame_of(V3<R>{q5.d, q6.a, q6.b}, V3<R>{q6.c, q6.d, q7.a});
+          const cx<R> T[4] = {{q7.b, q7.c}, {q7.d, q8.a}, {q8.b, q8.c}, {q8.d, q9.a}};
+          const V3<R> d{q9.b, q9.c, q9.d};
           cx<R> rm, rn;
           reverse_field(lc, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
           const R arg =

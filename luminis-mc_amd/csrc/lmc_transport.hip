@@ -378,7 +378,7 @@ void drop_graph(lmc_scene* s) {
 }
 
 template <class R> int ensure_queue(lmc_scene* s, int64_t cap) {
-  const size_t rec = sizeof(R) * kRecF + sizeof(double) + sizeof(int32_t) * kRecI;
+  const size_t rec = sizeof(R) * 4 * kRecQ + sizeof(double) + sizeof(int32_t) * kRecI;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
     const int64_t fit = static_cast<int64_t>((free_b + (s->queue.p ? s->queue.n : 0)) / 4 / rec);
@@ -509,8 +509,8 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     if (s->q_cap - reserve < per_photon) return fail(LMC_ERR_NOMEM, "crossing record queue too small");
     chunk = std::max<int64_t>(1, (s->q_cap - reserve) / per_photon);
     p.qf = reinterpret_cast<R*>(s->queue.p);
-    p.qd = reinterpret_cast<double*>(s->queue.p + sizeof(R) * kRecF * s->q_cap);
-    p.qi = reinterpret_cast<int32_t*>(s->queue.p + (sizeof(R) * kRecF + sizeof(double)) * s->q_cap);
+    p.qd = reinterpret_cast<double*>(s->queue.p + sizeof(R) * 4 * kRecQ * s->q_cap);
+    p.qi = reinterpret_cast<int32_t*>(s->queue.p + (sizeof(R) * 4 * kRecQ + sizeof(double)) * s->q_cap);
     p.q_cap = s->q_cap;
   }
   p.q_count = s->qcount.p;
