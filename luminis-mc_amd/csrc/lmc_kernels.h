@@ -26,17 +26,15 @@ template <class R> struct LaunchConst {
   int32_t source, temporal, launch_layer;
 };
 
+// Field order matters: the backend merges neighbouring kernel-argument loads
+// into wide scalar loads whose registers live (or spill) together, so the
+// pointers the event loop reads stay apart from those only the prologue
+// copies (layers, species, sensors, planes, laser constants).
 template <class R> struct KParams {
-  const LayerDev<R>* layers;
-  const SpeciesDev<R>* species;
   const R* cdf;
   const uint16_t* guide;
   const STab<R>* stab;
   const R* sel;
-  const SensorDev* sensors;
-  const PlaneDev* planes;
-  const LaunchConst<R>* lcs;     // one per scene
-  const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
   double* out;
   unsigned long long* hits;
   unsigned long long* counter;
@@ -55,6 +53,13 @@ template <class R> struct KParams {
   int32_t lds_grid, max_events, claim, q_chunk;
   int32_t lds_grad;             // adjoint runs: privatised grids keep their gradient rows in LDS too
   uint32_t key0, key1, idx_hi;
+  // staged into LDS by the prologue (read in the loop only by batched runs)
+  const LayerDev<R>* layers;
+  const SpeciesDev<R>* species;
+  const SensorDev* sensors;
+  const PlaneDev* planes;
+  const LaunchConst<R>* lcs;     // one per scene
+  const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
 };
 
 // One queued plane crossing accepted by at least one sensor of its plane:
@@ -923,26 +928,45 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // scenes then stay in global memory (cache resident: the index order keeps
 // only a few scenes in flight at a time) and each photon launches with its
 // own scene's laser.  Otherwise the scene constants are staged in LDS.
+// Fixed LDS layout of the transport kernel's scene header: compile-time
+// offsets from the LDS base, so the event loop addresses the staged layers,
+// species, sensors and planes with immediate offsets instead of keeping their
+// addresses in scalar registers.  Staged tables (when they fit) follow `end`.
+constexpr size_t al16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
+template <class R> struct LdsHdr {
+  static constexpr size_t hits = 0;
+  static constexpr size_t wq_base = al16(hits + sizeof(unsigned) * (kMaxSensors + 2));
+  static constexpr size_t wq_used = al16(wq_base + sizeof(unsigned long long) * 8);
+  static constexpr size_t common_end = al16(wq_used + sizeof(unsigned) * 8);  // batched runs stop here
+  static constexpr size_t layers = common_end;
+  static constexpr size_t species = al16(layers + sizeof(LayerDev<R>) * kMaxLayers);
+  static constexpr size_t sensors = al16(species + sizeof(SpeciesDev<R>) * kMaxSpecies);
+  static constexpr size_t planes = al16(sensors + sizeof(SensorDev) * kMaxSensors);
+  static constexpr size_t lc = al16(planes + sizeof(PlaneDev) * kMaxSensors);
+  static constexpr size_t end = al16(lc + sizeof(LaunchConst<R>));
+};
+
+// The transport kernel.  kBatch: the photons of one launch belong to several
+// scenes, concatenated in photon-index order (scene i owns global indices
+// [prefix[i], prefix[i+1])); layers, species, sensors and tables of all
+// scenes then stay in global memory (cache resident: the index order keeps
+// only a few scenes in flight at a time) and each photon launches with its
+// own scene's laser.  Otherwise the scene constants are staged in LDS.
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
 __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  unsigned char* q = smem;
-  auto carve = [&](size_t bytes) {
-    unsigned char* r = q;
-    q += (bytes + 15) & ~static_cast<size_t>(15);
-    return r;
-  };
+  using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
   auto copy = [&](void* dst, const void* src, size_t bytes) {
     const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
     uint32_t* d = reinterpret_cast<uint32_t*>(dst);
     for (int i = tid; i < static_cast<int>(bytes / 4); i += nt) d[i] = s[i];
   };
-  unsigned* sHits = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * (kMaxSensors + 2)));
+  unsigned* sHits = reinterpret_cast<unsigned*>(smem + H::hits);
   for (int i = tid; i < kMaxSensors + 2; i += nt) sHits[i] = 0u;
   const int n_waves = (nt + 63) >> 6;
-  unsigned long long* sWB = reinterpret_cast<unsigned long long*>(carve(sizeof(unsigned long long) * n_waves));
-  unsigned* sWU = reinterpret_cast<unsigned*>(carve(sizeof(unsigned) * n_waves));
+  unsigned long long* sWB = reinterpret_cast<unsigned long long*>(smem + H::wq_base);
+  unsigned* sWU = reinterpret_cast<unsigned*>(smem + H::wq_used);
   for (int i = tid; i < n_waves; i += nt) {
     sWB[i] = 0ull;
     sWU[i] = static_cast<unsigned>(p.q_chunk);  // full: the first crossing takes a chunk
@@ -953,7 +977,14 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
   v.wq_base = sWB;
   v.wq_used = sWU;
   uint64_t* sPre = nullptr;
+  unsigned char* q;
+  auto carve = [&](size_t bytes) {
+    unsigned char* r = q;
+    q += al16(bytes);
+    return r;
+  };
   if constexpr (kBatch) {
+    q = smem + H::common_end;
     sPre = reinterpret_cast<uint64_t*>(carve(sizeof(uint64_t) * (p.n_scenes + 1)));
     for (int i = tid; i <= p.n_scenes; i += nt) sPre[i] = p.prefix[i];
     v.lc = p.lcs;
@@ -962,11 +993,11 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
     v.sens = p.sensors;
     v.planes = p.planes;
   } else {
-    LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(carve(sizeof(LayerDev<R>) * p.n_layers));
-    SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(carve(sizeof(SpeciesDev<R>) * p.n_species));
-    SensorDev* sSen = reinterpret_cast<SensorDev*>(carve(sizeof(SensorDev) * p.n_sensors));
-    PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
-    LaunchConst<R>* sLC = reinterpret_cast<LaunchConst<R>*>(carve(sizeof(LaunchConst<R>)));
+    LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(smem + H::layers);
+    SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(smem + H::species);
+    SensorDev* sSen = reinterpret_cast<SensorDev*>(smem + H::sensors);
+    PlaneDev* sPl = reinterpret_cast<PlaneDev*>(smem + H::planes);
+    LaunchConst<R>* sLC = reinterpret_cast<LaunchConst<R>*>(smem + H::lc);
     copy(sL, p.layers, sizeof(LayerDev<R>) * p.n_layers);
     copy(sS, p.species, sizeof(SpeciesDev<R>) * p.n_species);
     copy(sSen, p.sensors, sizeof(SensorDev) * p.n_sensors);
@@ -977,6 +1008,7 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
     v.S = sS;
     v.sens = sSen;
     v.planes = sPl;
+    q = smem + H::end;
   }
   if constexpr (kLdsTables) {
     R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));

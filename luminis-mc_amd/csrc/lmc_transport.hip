@@ -466,20 +466,10 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   }
   if (st) std::memset(st, 0, sizeof(*st));
   if (total == 0) return LMC_OK;
-  size_t base_lds = 0;
-  auto add = [&](size_t b) { base_lds += (b + 15) & ~static_cast<size_t>(15); };
-  add(sizeof(unsigned) * (kMaxSensors + 2));
-  add(sizeof(unsigned long long) * 8);  // per-wave queue chunk state, up to 8 waves
-  add(sizeof(unsigned) * 8);
-  if (batch) {
-    add(sizeof(uint64_t) * (n_sc + 1));
-  } else {
-    add(sizeof(LayerDev<R>) * p.n_layers);
-    add(sizeof(SpeciesDev<R>) * p.n_species);
-    add(sizeof(SensorDev) * p.n_sensors);
-    add(sizeof(PlaneDev) * p.n_planes);
-    add(sizeof(LaunchConst<R>));
-  }
+  // LDS of the transport kernel: the fixed scene header (LdsHdr), then the
+  // photon-index prefix of a batch or, when they fit, the staged tables
+  const size_t base_lds = batch ? LdsHdr<R>::common_end + al16(sizeof(uint64_t) * (n_sc + 1)) : LdsHdr<R>::end;
+  if (p.n_planes > kMaxSensors && !batch) return fail(LMC_ERR_INVALID, "too many sensor planes");
   const size_t tab_lds = ((sizeof(R) * T.cdf_total + 15) & ~15ull) +
                          ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
                          ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
