@@ -936,8 +936,8 @@ constexpr size_t al16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
 template <class R> struct LdsHdr {
   static constexpr size_t hits = 0;
   static constexpr size_t wq_base = al16(hits + sizeof(unsigned) * (kMaxSensors + 2));
-  static constexpr size_t wq_used = al16(wq_base + sizeof(unsigned long long) * 8);
-  static constexpr size_t common_end = al16(wq_used + sizeof(unsigned) * 8);  // batched runs stop here
+  static constexpr size_t wq_used = al16(wq_base + sizeof(unsigned long long) * 16);  // up to 16 waves
+  static constexpr size_t common_end = al16(wq_used + sizeof(unsigned) * 16);  // batched runs stop here
   static constexpr size_t layers = common_end;
   static constexpr size_t species = al16(layers + sizeof(LayerDev<R>) * kMaxLayers);
   static constexpr size_t sensors = al16(species + sizeof(SpeciesDev<R>) * kMaxSpecies);
@@ -952,8 +952,11 @@ template <class R> struct LdsHdr {
 // scenes then stay in global memory (cache resident: the index order keeps
 // only a few scenes in flight at a time) and each photon launches with its
 // own scene's laser.  Otherwise the scene constants are staged in LDS.
+// The body is shared by two entry points: 512-thread workgroups (two per CU
+// when their LDS allows) and 1024-thread workgroups (one per CU, the whole
+// LDS for tables that two workgroups could not hold).
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
-__global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
+__device__ __forceinline__ void transport_body(const KParams<R>& p) {
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1100,6 +1103,16 @@ __global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KPa
   __syncthreads();
   if (tid == 0) atomicAdd(&p.stats[3], static_cast<unsigned long long>(sHits[kMaxSensors]));
   if (lane == 0) atomicAdd(&p.stats[1], static_cast<unsigned long long>(w_ev));
+}
+
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
+__global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
+  transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch>(p);
+}
+
+template <class R, bool kXo, bool kTrack, bool kAdj>
+__global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
+  transport_body<R, kXo, kTrack, true, kAdj, false>(p);
 }
 
 // Sensor kernel: one thread per queued crossing.  For every sensor that

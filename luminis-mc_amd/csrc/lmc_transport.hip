@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -177,6 +178,9 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   std::vector<uint16_t> guide;
   std::vector<STab<R>> stab;
   std::vector<R> sel;
+  // species with identical tables (e.g. equal media in several layers) share one copy
+  std::map<std::vector<double>, std::pair<int32_t, int32_t>> cdf_seen;  // -> (cdf_off, guide_off)
+  std::map<std::vector<double>, int32_t> s_seen;                       // -> s_off
   for (size_t i = 0; i < s->species.size(); ++i) {
     const lmc_species_desc& d = s->species[i];
     SpeciesDev<R>& o = sp[i];
@@ -188,26 +192,44 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     if (d.phase_kind == LMC_PHASE_TABLE_THETA || d.phase_kind == LMC_PHASE_TABLE_COS) {
       if (d.cdf_n < 2 || d.cdf_n > 65535) return fail(LMC_ERR_INVALID, "cdf_n must be in [2, 65535]");
       o.cdf_n = d.cdf_n;
-      o.cdf_off = static_cast<int32_t>(cdf.size());
-      const double* c = s->tables.data() + d.cdf_offset;
-      std::vector<R> cr(d.cdf_n);
-      for (int j = 0; j < d.cdf_n; ++j) cr[j] = static_cast<R>(c[j]);
-      cdf.insert(cdf.end(), cr.begin(), cr.end());
-      // guide g[j] = lower_bound(cr, j/G) on the converted values; g[G] = n-1
-      o.guide_off = static_cast<int32_t>(guide.size());
       o.guide_n = kGuideN;
-      int idx = 0;
-      for (int j = 0; j < kGuideN; ++j) {
-        const R thr = static_cast<R>(static_cast<double>(j) / kGuideN);
-        while (idx < d.cdf_n - 1 && cr[idx] < thr) ++idx;
-        guide.push_back(static_cast<uint16_t>(idx));
+      const double* c = s->tables.data() + d.cdf_offset;
+      std::vector<double> key(c, c + d.cdf_n);
+      auto hit = cdf_seen.find(key);
+      if (hit != cdf_seen.end()) {
+        o.cdf_off = hit->second.first;
+        o.guide_off = hit->second.second;
+      } else {
+        o.cdf_off = static_cast<int32_t>(cdf.size());
+        o.guide_off = static_cast<int32_t>(guide.size());
+        cdf_seen.emplace(std::move(key), std::make_pair(o.cdf_off, o.guide_off));
+        std::vector<R> cr(d.cdf_n);
+        for (int j = 0; j < d.cdf_n; ++j) cr[j] = static_cast<R>(c[j]);
+        cdf.insert(cdf.end(), cr.begin(), cr.end());
+        // guide g[j] = lower_bound(cr, j/G) on the converted values; g[G] = n-1
+        int idx = 0;
+        for (int j = 0; j < kGuideN; ++j) {
+          const R thr = static_cast<R>(static_cast<double>(j) / kGuideN);
+          while (idx < d.cdf_n - 1 && cr[idx] < thr) ++idx;
+          guide.push_back(static_cast<uint16_t>(idx));
+        }
+        guide.push_back(static_cast<uint16_t>(d.cdf_n - 1));
       }
-      guide.push_back(static_cast<uint16_t>(d.cdf_n - 1));
     }
     if (d.s_n < 2) return fail(LMC_ERR_INVALID, "species S table needs >= 2 nodes");
     o.s_n = d.s_n;
-    o.s_off = static_cast<int32_t>(stab.size());
     o.inv_ds = static_cast<R>(1.0 / d.s_dtheta);
+    {
+      std::vector<double> key(s->tables.data() + d.s1_offset, s->tables.data() + d.s1_offset + 2 * d.s_n);
+      key.insert(key.end(), s->tables.data() + d.s2_offset, s->tables.data() + d.s2_offset + 2 * d.s_n);
+      auto hit = s_seen.find(key);
+      if (hit != s_seen.end()) {
+        o.s_off = hit->second;
+        continue;
+      }
+      s_seen.emplace(std::move(key), static_cast<int32_t>(stab.size()));
+    }
+    o.s_off = static_cast<int32_t>(stab.size());
     for (int j = 0; j < d.s_n; ++j) {
       const double* a = s->tables.data() + d.s1_offset + 2 * j;
       const double* b = s->tables.data() + d.s2_offset + 2 * j;
@@ -474,17 +496,22 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
                          ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
                          ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
   // Launch shape: 16 resident waves per CU (4 per SIMD at <= 128 VGPRs).  The
-  // tables go to LDS when two 512-thread workgroups per CU can hold them;
-  // otherwise 256-thread workgroups read them through the caches.
-  const bool fit512 = !batch && 2 * (base_lds + tab_lds) <= g_max_lds_cu && base_lds + tab_lds <= g_max_lds_block;
-  int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : 256);
-  if (block % 64 != 0 || block > 512) return fail(LMC_ERR_INVALID, "block_size must be a multiple of 64, <= 512");
+  // tables go to LDS when two 512-thread workgroups per CU can hold them, or
+  // else (FP32 forward runs) when one 1024-thread workgroup can; otherwise
+  // 256-thread workgroups read them through the caches.
+  const size_t need = base_lds + tab_lds;
+  const bool fit512 = !batch && 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
+  const bool wide_ok = !batch && !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
+  int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : (wide_ok ? 1024 : 256));
+  if (block % 64 != 0 || block > 1024) return fail(LMC_ERR_INVALID, "block_size must be a multiple of 64, <= 1024");
+  if (block > 512 && !wide_ok)
+    return fail(LMC_ERR_UNSUPPORTED, "1024-thread workgroups need FP32 forward runs with LDS-resident tables");
   const int bpc = r->blocks_per_cu > 0 ? r->blocks_per_cu : std::max(1, 1024 / block);
   const int grid = g_num_cu * bpc;
-  const bool lds_tables = !batch &&
-                          (base_lds + tab_lds) * static_cast<size_t>(std::min(bpc, 1024 / block)) <= g_max_lds_cu &&
-                          base_lds + tab_lds <= g_max_lds_block;
-  const size_t lds = lds_tables ? base_lds + tab_lds : base_lds;
+  const bool lds_tables = block > 512 || (!batch && need * static_cast<size_t>(std::min(bpc, std::max(1, 1024 / block))) <=
+                                                         g_max_lds_cu &&
+                                          need <= g_max_lds_block);
+  const size_t lds = lds_tables ? need : base_lds;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   const bool track = r->track_reverse_paths != 0;
   int64_t chunk = static_cast<int64_t>(total);
@@ -507,10 +534,14 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   void (*kt)(KParams<R>) = nullptr;
   if (batch)
     kt = track ? k_transport<R, kXo, true, false, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
-  else if (track)
+  else if (block > 512) {
+    if constexpr (sizeof(R) == 4 && !kAdj)
+      kt = track ? k_transport_wide<R, kXo, true, kAdj> : k_transport_wide<R, kXo, false, kAdj>;
+  } else if (track)
     kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, false> : k_transport<R, kXo, true, false, kAdj, false>;
   else
     kt = lds_tables ? k_transport<R, kXo, false, true, kAdj, false> : k_transport<R, kXo, false, false, kAdj, false>;
+  if (!kt) return fail(LMC_ERR_UNSUPPORTED, "no transport kernel for this launch shape");
   int rc = raise_lds_limit(kt, lds);
   if (rc) return rc;
   void (*kr)(KParams<R>) = k_crossings<R, kAdj>;
