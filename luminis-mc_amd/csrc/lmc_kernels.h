@@ -141,8 +141,9 @@ template <class R, bool kXo> struct Photon {
   cx<R> b2, b1;          // last event's J_n = diag(b2, b1) R(bc, bs), pending
   R bc, bs;
   int32_t sp;            // (first species << 16) | last species
-  // Philox draws of the current event kept past the flight
-  R u_sp, u_th, u_rr;
+  // Philox words of the current event kept past the flight (theta, first
+  // azimuth trial, the shared 24-bit auxiliary draw)
+  uint32_t w_th, w_pa, w_aa;
   XoState<kXo> xo;
   R aL[kMaxGradLayers], aN[kMaxGradLayers], aG[kMaxGradLayers];
 };
@@ -160,6 +161,22 @@ __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t
   asm volatile("v_mov_b32 %0, %1" : "=v"(k0v) : "s"(key0));
   asm volatile("v_mov_b32 %0, %1" : "=v"(k1v) : "s"(key1));
   return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), k0v, k1v);
+}
+
+// Philox draw map of one event e (counter (photon, high word, e, sub)):
+//   sub 0: word x -> free path (high 24 bits), y -> theta (high 24),
+//          z, w -> first azimuth trial (high 21 bits each), and one auxiliary
+//          24-bit uniform from the low bytes of x, y, z (species choice in a
+//          mixture layer, otherwise the roulette draw);
+//   sub k >= 1: azimuth trials 3k-1 .. 3k+1 (21-bit uniforms);
+//   sub 0xFFFFFFFE: roulette draw of an event whose auxiliary draw chose a species.
+// 21-bit uniform on the open interval (0,1).
+template <class R> __device__ __forceinline__ R u01_21(uint32_t x21) {
+  return static_cast<R>(static_cast<float>(x21) + 0.5f) * static_cast<R>(4.76837158203125e-7);
+}
+template <class R, bool kXo> __device__ __forceinline__ R aux_uniform(const Photon<R, kXo>& ph) {
+  const uint32_t x = ((ph.w_th & 0xFFu) << 16) | (((ph.w_pa >> 3) & 0xFFu) << 8) | ((ph.w_aa >> 3) & 0xFFu);
+  return static_cast<R>(static_cast<float>(x) + 0.5f) * static_cast<R>(5.9604644775390625e-8);
 }
 
 // Inverse-CDF lookup: SamplingTable::Sample (src/table.cpp:50-66).  The
@@ -675,9 +692,9 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   } else {
     const uint4 b = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), 0u);
     u_fp = u01<R>(b.x);
-    ph.u_sp = u01<R>(b.y);
-    ph.u_th = u01<R>(b.z);
-    ph.u_rr = u01<R>(b.w);
+    ph.w_th = (b.y & 0xFFFFFF00u) | (b.x & 0xFFu);  // theta: high 24 bits; low byte: aux bits 16-23
+    ph.w_pa = (b.z & 0xFFFFF800u) | ((b.y & 0xFFu) << 3);  // trial-1 angle: high 21 bits; aux bits 8-15
+    ph.w_aa = (b.w & 0xFFFFF800u) | ((b.z & 0xFFu) << 3);  // trial-1 accept: high 21 bits; aux bits 0-7
   }
   R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
   const V3<R> dir = ph.P.s;
@@ -752,7 +769,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     if constexpr (kXo)
       u = static_cast<R>(ph.xo.s.uniform());
     else
-      u = ph.u_sp;
+      u = aux_uniform<R>(ph);
     const R* c = v.sel + Ld.sel_off;
     int lo = 0, hi = Ld.sp_count;
     while (lo < hi) {
@@ -770,7 +787,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   if constexpr (kXo)
     uth = static_cast<R>(ph.xo.s.uniform());
   else
-    uth = ph.u_th;
+    uth = u01<R>(ph.w_th);
   R theta;
   const int kind = S.kind;
   if (kind == LMC_PHASE_HG) {  // phase.cpp:108-115
@@ -806,41 +823,38 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   const R Fmax = R(0.5) * (a + b) + R(0.5) * M<R>::sqrt(diff * diff + c * c);
   R cph = R(1), sph = R(0), F = R(0);
   int trials = 0;
-  for (; trials < 256;) {
-    R pa, aa, pb, ab;
-    if constexpr (kXo) {
-      pa = static_cast<R>(ph.xo.s.uniform());
-      aa = static_cast<R>(ph.xo.s.uniform());
-      pb = aa;
-      ab = aa;
-    } else {
-      const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events),
-                                    1u + static_cast<uint32_t>(trials >> 1));
-      pa = u01<R>(blk.x);
-      aa = u01<R>(blk.y);
-      pb = u01<R>(blk.z);
-      ab = u01<R>(blk.w);
-    }
+  auto trial = [&](R pa, R aa) -> bool {
     R sp_, cp_;
     M<R>::sincos_rev(pa, &sp_, &cp_);
-    R Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
+    const R Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
     ++trials;
     if (aa * Fmax <= (Ft > R(0) ? Ft : R(0))) {
       cph = cp_;
       sph = sp_;
       F = Ft;
-      break;
+      return true;
     }
-    if constexpr (kXo) continue;
-    // second trial of the Philox pair
-    M<R>::sincos_rev(pb, &sp_, &cp_);
-    Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
-    ++trials;
-    if (ab * Fmax <= (Ft > R(0) ? Ft : R(0))) {
-      cph = cp_;
-      sph = sp_;
-      F = Ft;
-      break;
+    return false;
+  };
+  if constexpr (kXo) {
+    while (trials < 256) {
+      const R pa = static_cast<R>(ph.xo.s.uniform());
+      const R aa = static_cast<R>(ph.xo.s.uniform());
+      if (trial(pa, aa)) break;
+    }
+  } else {
+    // trial 1 from the event's main block, then three trials per extra block
+    // (21-bit uniforms: the angle's and the test's high 21 bits of a word; the
+    // third trial joins the four words' low 11 bits)
+    if (!trial(u01_21<R>(ph.w_pa >> 11), u01_21<R>(ph.w_aa >> 11))) {
+      for (uint32_t k = 1; trials < 256; ++k) {
+        const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), k);
+        if (trial(u01_21<R>(blk.x >> 11), u01_21<R>(blk.y >> 11))) break;
+        if (trial(u01_21<R>(blk.z >> 11), u01_21<R>(blk.w >> 11))) break;
+        if (trial(u01_21<R>(((blk.x & 0x7FFu) << 10) | ((blk.y & 0x7FFu) >> 1)),
+                  u01_21<R>(((blk.z & 0x7FFu) << 10) | ((blk.w & 0x7FFu) >> 1))))
+          break;
+      }
     }
   }
   // frame update P <- A(theta, phi) P (simulation.cpp:438-453)
@@ -903,10 +917,15 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   if (Ld.mu_t > R(0)) ph.w *= Ld.albedo;
   if (ph.w < static_cast<R>(1e-4)) {
     R ur;
-    if constexpr (kXo)
+    if constexpr (kXo) {
       ur = static_cast<R>(ph.xo.s.uniform());
-    else
-      ur = ph.u_rr;
+    } else if (Ld.sp_count > 1) {  // the auxiliary draw went to the species choice
+      const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events - 1),
+                                    0xFFFFFFFEu);
+      ur = u01<R>(blk.x);
+    } else {
+      ur = aux_uniform<R>(ph);
+    }
     if (ur < static_cast<R>(0.1))
       ph.w /= static_cast<R>(0.1);
     else
@@ -1105,8 +1124,12 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   if (lane == 0) atomicAdd(&p.stats[1], static_cast<unsigned long long>(w_ev));
 }
 
+// FP32 forward runs: at most 128 VGPRs, so 16 waves of a CU stay resident
+// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
-__global__ void __launch_bounds__(512, (sizeof(R) == 4 ? 2 : 1)) k_transport(KParams<R> p) {
+__global__ void __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (kAdj ? 3 : 4) : 2)))
+k_transport(KParams<R> p) {
   transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch>(p);
 }
 

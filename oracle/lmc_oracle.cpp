@@ -1159,6 +1159,46 @@ void oracle_xoshiro_u64(uint64_t state_seed, int n, uint64_t* out) {
 uint64_t oracle_mix_seed(uint64_t base, uint64_t tid) { return mix_seed(base, tid); }
 
 // Philox4x32-10 (Salmon et al., SC'11), host restatement for known answers.
+// reverse_field on explicit inputs (test hook): frames P0, P1, P_{n-1}, P_n
+// (rows m, n, s; 9 doubles each), raw interior product T (4 complex, row
+// major, re/im interleaved), launch Jones vector E0 (re/im of m, n), and the
+// S1/S2 tables (n_s nodes on theta_i = i*dtheta, re/im interleaved) of the
+// last- and first-vertex species.  out: the normalised reverse field (re/im of
+// m, n) in the exit basis.
+void oracle_reverse_field(const double* P0, const double* P1, const double* Pnm1, const double* Pn,
+                          const double* T, const double* E0, int n_s, double dtheta, const double* s1_last,
+                          const double* s2_last, const double* s1_first, const double* s2_first, double* out) {
+  Scene S;
+  S.species.resize(2);
+  for (int k = 0; k < 2; ++k) {
+    Species& sp = S.species[k];
+    std::memset(&sp.d, 0, sizeof(sp.d));
+    sp.inv_ds = 1.0 / dtheta;
+    const double* a = k == 0 ? s1_last : s1_first;
+    const double* b = k == 0 ? s2_last : s2_first;
+    for (int i = 0; i < n_s; ++i) {
+      sp.s1.push_back(cplx(a[2 * i], a[2 * i + 1]));
+      sp.s2.push_back(cplx(b[2 * i], b[2 * i + 1]));
+    }
+  }
+  Photon ph;
+  for (int i = 0; i < 9; ++i) {
+    ph.P0.a[i] = P0[i];
+    ph.P1.a[i] = P1[i];
+    ph.Pn1.a[i] = Pnm1[i];
+    ph.Pn.a[i] = Pn[i];
+  }
+  for (int i = 0; i < 4; ++i) ph.Traw.a[i] = cplx(T[2 * i], T[2 * i + 1]);
+  ph.E0 = {cplx(E0[0], E0[1]), cplx(E0[2], E0[3])};
+  ph.last_sp = 0;
+  ph.first_sp = 1;
+  const CV E = reverse_field(S, ph);
+  out[0] = E.m.real();
+  out[1] = E.m.imag();
+  out[2] = E.n.real();
+  out[3] = E.n.imag();
+}
+
 void oracle_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
   uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
   for (int r = 0; r < 10; ++r) {

@@ -44,6 +44,8 @@ def lib():
         L.oracle_xoshiro_u64.argtypes = [ctypes.c_uint64, ctypes.c_int, _u64p]
         L.oracle_mix_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.oracle_mix_seed.restype = ctypes.c_uint64
+        L.oracle_reverse_field.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                           _dp, _dp, _dp]
         L.oracle_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32)]
         _lib = L
@@ -102,6 +104,26 @@ def xoshiro_u64(state_seed: int, n: int) -> np.ndarray:
 
 def mix_seed(base: int, tid: int) -> int:
     return int(lib().oracle_mix_seed(int(base), int(tid)))
+
+
+def reverse_field(P0, P1, Pnm1, Pn, T, E0, s_last, s_first, dtheta: float):
+    """The restatement's reverse_field on explicit inputs.  P*: 3x3 frames (rows
+    m, n, s); T: 2x2 complex raw interior product; E0: launch Jones vector;
+    s_last / s_first: (S1, S2) complex tables on theta_i = i*dtheta of the last-
+    and first-vertex species.  Returns the normalised reverse field (2 complex)
+    in the exit basis."""
+    def c(a):
+        a = np.asarray(a, dtype=np.complex128).ravel()
+        return np.ascontiguousarray(np.stack([a.real, a.imag], axis=1).ravel())
+
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (P0, P1, Pnm1, Pn)]
+    Tc, Ec = c(T), c(E0)
+    s1l, s2l, s1f, s2f = c(s_last[0]), c(s_last[1]), c(s_first[0]), c(s_first[1])
+    out = np.zeros(4)
+    lib().oracle_reverse_field(*(a.ctypes.data_as(_dp) for a in arrs), Tc.ctypes.data_as(_dp), Ec.ctypes.data_as(_dp),
+                               int(len(s_last[0])), float(dtheta), s1l.ctypes.data_as(_dp), s2l.ctypes.data_as(_dp),
+                               s1f.ctypes.data_as(_dp), s2f.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return np.array([out[0] + 1j * out[1], out[2] + 1j * out[3]])
 
 
 def philox(ctr, k0: int, k1: int):

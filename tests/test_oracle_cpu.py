@@ -130,3 +130,90 @@ def test_sampling_cdf_closed_forms():
     c = oracle.sampling_cdf(n, 0.0, 1.0, np.linspace(0, 1, n + 1))
     k = np.arange(n + 1)
     assert np.allclose(c, k * (k + 1) / (n * (n + 1)), atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# reverse_field against brute force: the Xu three-stage reconstruction of the
+# time-reversed partner (detector.cpp:1080-1161) must equal explicit
+# propagation of the reversed vertex sequence with the forward scattering rules
+# (frame update A(theta, phi) P and Jones S R(phi), simulation.cpp:438-492).
+
+
+def _s_tables(n=20001):
+    th = np.linspace(0.0, np.pi, n)
+    s2 = np.cos(th) * (1.0 + 0.3j * np.sin(th)) + 0.1 * np.exp(0.7j * th)
+    s1 = 1.0 + 0.2j * th - 0.15 * th ** 2
+    return th, s1, s2
+
+
+def _lerp_c(th_grid, y, th):
+    return np.interp(th, th_grid, y.real) + 1j * np.interp(th, th_grid, y.imag)
+
+
+def _bf_propagate(P0, E0, dirs, tabs):
+    th_grid, s1, s2 = tabs
+    P, E = P0.copy(), E0.astype(complex).copy()
+    frames, Js = [P0.copy()], []
+    for d in dirs:
+        m, n, s = P
+        th = np.arccos(np.clip(s @ d, -1.0, 1.0))
+        ph = np.arctan2(d @ n, d @ m)
+        c, sn, ct, st = np.cos(ph), np.sin(ph), np.cos(th), np.sin(th)
+        A = np.array([[ct * c, ct * sn, -st], [-sn, c, 0.0], [st * c, st * sn, ct]])
+        J = np.diag([_lerp_c(th_grid, s2, th), _lerp_c(th_grid, s1, th)]) @ np.array([[c, sn], [-sn, c]])
+        E = J @ E
+        P = A @ P
+        frames.append(P)
+        Js.append(J)
+    return frames, E, Js
+
+
+@pytest.mark.parametrize("n_scatter", [2, 3, 4, 6])
+def test_reverse_field_equals_reversed_path(n_scatter):
+    rng = np.random.default_rng(100 + n_scatter)
+    tabs = _s_tables()
+    dth = tabs[0][1] - tabs[0][0]
+    for trial in range(20):
+        P0 = np.eye(3)
+        E0 = rng.normal(size=2) + 1j * rng.normal(size=2)
+        E0 /= np.linalg.norm(E0)
+        dirs = [v / np.linalg.norm(v) for v in rng.normal(size=(n_scatter, 3))]
+        frames, Ef, Js = _bf_propagate(P0, E0, dirs, tabs)
+        T = np.eye(2, dtype=complex)
+        for J in Js[1:n_scatter - 1]:  # interior product J_{n-1} ... J_2
+            T = J @ T
+        Pn = frames[n_scatter]
+        Er = oracle.reverse_field(P0, frames[1], frames[n_scatter - 1], Pn, T, E0, (tabs[1], tabs[2]),
+                                  (tabs[1], tabs[2]), dth)
+        # reversed partner: s0 -> -s_{n-1} -> ... -> -s_1 -> s_n
+        s = [f[2] for f in frames]
+        rdirs = [-s[k] for k in range(n_scatter - 1, 0, -1)] + [s[n_scatter]]
+        rframes, Erb, _ = _bf_propagate(P0, E0, rdirs, tabs)
+        lab_bf = Erb[0] * rframes[-1][0] + Erb[1] * rframes[-1][1]
+        lab_bf /= np.linalg.norm(lab_bf)
+        lab_or = Er[0] * Pn[0] + Er[1] * Pn[1]
+        assert abs(np.linalg.norm(lab_or) - 1.0) < 1e-12
+        assert np.allclose(lab_or, lab_bf, atol=1e-9), (n_scatter, trial)
+
+
+def test_reverse_field_catches_a_wrong_interior_order():
+    """The brute-force comparison is sensitive: the interior product in the
+    wrong order (J_2 ... J_{n-1}) gives a different reverse field."""
+    rng = np.random.default_rng(7)
+    tabs = _s_tables()
+    dth = tabs[0][1] - tabs[0][0]
+    P0 = np.eye(3)
+    E0 = np.array([1.0, 0.3j]) / np.linalg.norm([1.0, 0.3])
+    dirs = [v / np.linalg.norm(v) for v in rng.normal(size=(5, 3))]
+    frames, Ef, Js = _bf_propagate(P0, E0, dirs, tabs)
+    T_wrong = np.eye(2, dtype=complex)
+    for J in Js[1:4]:
+        T_wrong = T_wrong @ J
+    Er = oracle.reverse_field(P0, frames[1], frames[4], frames[5], T_wrong, E0, (tabs[1], tabs[2]),
+                              (tabs[1], tabs[2]), dth)
+    s = [f[2] for f in frames]
+    rframes, Erb, _ = _bf_propagate(P0, E0, [-s[4], -s[3], -s[2], -s[1], s[5]], tabs)
+    lab_bf = Erb[0] * rframes[-1][0] + Erb[1] * rframes[-1][1]
+    lab_bf /= np.linalg.norm(lab_bf)
+    lab_or = Er[0] * frames[5][0] + Er[1] * frames[5][1]
+    assert not np.allclose(lab_or, lab_bf, atol=1e-6)
