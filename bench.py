@@ -41,8 +41,8 @@ PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no pac
 # counters read for a device-to-device copy of known size (`bench.py
 # --copy-probe 256`): fetch x2.00, write x1.00.  Table reads are cache
 # resident; the writes are the crossing records handed to the sensor kernel.
-HBM_FETCH_B_PER_PHOTON = 1236 * 1024 * 2.00 / 4e6
-HBM_WRITE_B_PER_PHOTON = 2.844e6 * 1024 * 1.00 / 4e6
+HBM_FETCH_B_PER_PHOTON = 721.1 * 1024 * 2.00 / 4e6
+HBM_WRITE_B_PER_PHOTON = 6.371e5 * 1024 * 1.00 / 4e6
 
 
 def metric_name():
@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e8, help="photons per step")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -276,6 +276,7 @@ def main():
     tot = {"photons": 0, "events": 0, "phi_trials": 0, "crossings": 0, "cbs_detections": 0,
            "transport_ms": 0.0, "sensor_ms": 0.0, "kernel_ms": 0.0, "transport_launches": 0}
     t0 = time.perf_counter()
+    st = None
     for k in range(args.steps):
         st = scene.run(run_desc_from(cfg, n_photons=n, photon_offset=k * n))
         for key in tot:
@@ -293,7 +294,10 @@ def main():
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "frac_of_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED, 4), "traffic": round(traffic),
             "traffic_gb_s": round(traffic / avg_launch_s / 1e9, 1),
-            "kernel": "k_transport", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+            "kernel": "k_transport_wide" if st and st["block_size"] > 512 else "k_transport",
+            "launch": {"block": st["block_size"], "blocks_per_cu": st["blocks_per_cu"],
+                       "lds_tables": bool(st["lds_tables"])} if st else None,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 3),
             "flop_per_launch": per_launch}
     out = {
         "metric": metric_name(), "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,

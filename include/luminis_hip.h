@@ -193,6 +193,10 @@ typedef struct lmc_stats {
   double transport_ms;     /* device time of the transport kernel launches    */
   double sensor_ms;        /* device time of the sensor (crossing) kernel     */
   uint64_t transport_launches;
+  int32_t block_size;      /* launch shape the run used                      */
+  int32_t blocks_per_cu;
+  int32_t lds_tables;      /* 1: phase/amplitude tables staged in LDS         */
+  int32_t reserved0;
 } lmc_stats;
 
 /* Device facts used for launch shaping and roofline reporting. */

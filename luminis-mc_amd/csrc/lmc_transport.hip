@@ -667,6 +667,9 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     st->transport_ms = ms_t;  // graph runs: the whole graph
     st->sensor_ms = ms_r;
     st->transport_launches = seq.size();
+    st->block_size = block;
+    st->blocks_per_cu = bpc;
+    st->lds_tables = lds_tables ? 1 : 0;
   }
   return LMC_OK;
 }
