@@ -19,7 +19,8 @@ constexpr int kMaxGradLayers = 4;
 // ------------------------------------------------------------ math ------
 template <class R> struct M;
 template <> struct M<float> {
-  static __device__ __forceinline__ float log(float x) { return __logf(x); }
+  // hardware log2 (v_log_f32, ~1 ulp) scaled to ln
+  static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
   static __device__ __forceinline__ float sqrt(float x) { return __fsqrt_rn(x); }
   static __device__ __forceinline__ float rsqrt(float x) { return __frsqrt_rn(x); }
   static __device__ __forceinline__ float acos(float x) { return acosf(x); }

@@ -154,12 +154,12 @@ template <class R> __device__ __forceinline__ V3<R> cross3(V3<R> a, V3<R> b) {
 
 __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t key1, uint32_t idx, uint32_t idx_hi,
                                              uint32_t ev, uint32_t sub) {
-  // The key is uniform, but keeping it in vector registers stops the
-  // compiler from hoisting all 20 round keys of the schedule into scalar
-  // registers for the whole event loop (they would spill).
+  // The key is uniform: a scalar copy the compiler cannot see through keeps
+  // the round-key schedule on the scalar unit (SALU adds) without hoisting
+  // all 20 round keys into scalar registers for the whole event loop.
   uint32_t k0v, k1v;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(k0v) : "s"(key0));
-  asm volatile("v_mov_b32 %0, %1" : "=v"(k1v) : "s"(key1));
+  asm volatile("s_mov_b32 %0, %1" : "=s"(k0v) : "s"(key0));
+  asm volatile("s_mov_b32 %0, %1" : "=s"(k1v) : "s"(key1));
   return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), k0v, k1v);
 }
 
