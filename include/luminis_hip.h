@@ -186,7 +186,8 @@ typedef struct lmc_run_desc {
 typedef struct lmc_stats {
   uint64_t photons;        /* photons launched                                */
   uint64_t events;         /* scattering events                               */
-  uint64_t phi_trials;     /* azimuth rejection trials                        */
+  uint64_t phi_trials;     /* azimuth draws: rejection trials in the xoshiro
+                              check mode, one per event otherwise             */
   uint64_t crossings;      /* sensor-plane crossings processed                */
   uint64_t cbs_detections; /* photons deposited with a reverse path (events>=2) */
   double kernel_ms;        /* device time of all kernels of the run           */
@@ -289,6 +290,15 @@ int lmc_sampling_cdf(int32_t n_div, double min_val, double max_val,
 /* Philox4x32-10 on device: out[4*i..] = philox(ctr[4*i..], key). */
 int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
                      uint32_t* out);
+
+/* Azimuth sampler of the Philox path on device: for each i, the (cos phi,
+ * sin phi, F(phi)) that the transport draws from the two 32-bit words
+ * words[2i] (component and radius) and words[2i+1] (angle, half period) for
+ * F(phi) = a cos^2 + b sin^2 + c sin cos, (a, b, c) = abc[3i..3i+2] (the
+ * density of src/phase.cpp:27-49, sampled exactly as a uniform/cos^2 mixture
+ * about the principal axis instead of by rejection). */
+int lmc_probe_azimuth(int32_t n, int32_t precision, const double* abc, const uint32_t* words,
+                      double* out);
 
 #ifdef __cplusplus
 }

@@ -141,9 +141,9 @@ template <class R, bool kXo> struct Photon {
   cx<R> b2, b1;          // last event's J_n = diag(b2, b1) R(bc, bs), pending
   R bc, bs;
   int32_t sp;            // (first species << 16) | last species
-  // Philox words of the current event kept past the flight (theta, first
-  // azimuth trial, the shared 24-bit auxiliary draw)
-  uint32_t w_th, w_pa, w_aa;
+  // Philox words of the current event kept past the flight (theta, the two
+  // azimuth draws, the shared 24-bit auxiliary draw)
+  uint32_t w_th, w_ph, w_az;
   XoState<kXo> xo;
   R aL[kMaxGradLayers], aN[kMaxGradLayers], aG[kMaxGradLayers];
 };
@@ -163,20 +163,64 @@ __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t
   return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), k0v, k1v);
 }
 
-// Philox draw map of one event e (counter (photon, high word, e, sub)):
-//   sub 0: word x -> free path (high 24 bits), y -> theta (high 24),
-//          z, w -> first azimuth trial (high 21 bits each), and one auxiliary
-//          24-bit uniform from the low bytes of x, y, z (species choice in a
-//          mixture layer, otherwise the roulette draw);
-//   sub k >= 1: azimuth trials 3k-1 .. 3k+1 (21-bit uniforms);
-//   sub 0xFFFFFFFE: roulette draw of an event whose auxiliary draw chose a species.
-// 21-bit uniform on the open interval (0,1).
-template <class R> __device__ __forceinline__ R u01_21(uint32_t x21) {
-  return static_cast<R>(static_cast<float>(x21) + 0.5f) * static_cast<R>(4.76837158203125e-7);
-}
+// Philox draw map of one event e (counter (photon, high word, e, 0)): word x
+// -> free path (high 24 bits), y -> theta (high 24), z -> azimuth component
+// and radius (high 24), w -> azimuth angle (high 23) and half period (bit 8);
+// the low bytes of x, y, z form one auxiliary 24-bit uniform (the species
+// choice in a mixture layer, otherwise the roulette draw).  One block per
+// event; emission draws use event 0xFFFFFFFF.
 template <class R, bool kXo> __device__ __forceinline__ R aux_uniform(const Photon<R, kXo>& ph) {
-  const uint32_t x = ((ph.w_th & 0xFFu) << 16) | (((ph.w_pa >> 3) & 0xFFu) << 8) | ((ph.w_aa >> 3) & 0xFFu);
+  const uint32_t x = ((ph.w_th & 0xFFu) << 16) | ((ph.w_az & 0xFFu) << 8) | (ph.w_ph & 0xFFu);
   return static_cast<R>(static_cast<float>(x) + 0.5f) * static_cast<R>(5.9604644775390625e-8);
+}
+
+// a / b: hardware reciprocal in FP32 (~1 ulp), exact division in FP64.
+template <class R> __device__ __forceinline__ R qdiv(R a, R b) {
+  if constexpr (sizeof(R) == 4)
+    return a * __builtin_amdgcn_rcpf(b);
+  else
+    return a / b;
+}
+
+// Azimuth with density F(phi) = a cos^2 phi + b sin^2 phi + c sin phi cos phi,
+// the density the reference samples by rejection (phase.cpp:27-49), here
+// exactly and without a loop.  F is a positive semi-definite quadratic form
+// in (cos phi, sin phi) with eigenvalues l1 = M + Rm >= l2 = M - Rm (M =
+// (a+b)/2, Rm = |((a-b)/2, c/2)|), so F(phi0 + chi) = l2 + (l1 - l2) cos^2 chi
+// about the principal direction phi0: a mixture of the uniform density
+// (weight 1 - e) and cos^2 chi / pi (weight e = Rm / M).  The angle chi of a
+// point uniform in the unit disk centred at (1, 0) has density proportional to
+// cos^2 chi on (-pi/2, pi/2) (the disk is r < 2 cos chi in polar form), and a
+// half-period flip extends it to the whole circle.  Draws: v (bits 8-31 of the
+// first word) picks the component, v M < Rm, and is then uniform again as
+// v M / Rm, the disk point's squared radius; bits 9-31 of the second word are
+// the disk point's angle (the whole direction for the uniform component), bit
+// 8 the half period.  Returns cos phi, sin phi and F(phi).
+template <class R>
+__device__ __forceinline__ void azimuth_sample(R a, R b, R c, uint32_t wv, uint32_t wa, R& cph, R& sph, R& F) {
+  const R Mm = R(0.5) * (a + b), A = R(0.5) * (a - b), B = R(0.5) * c;
+  const R Rm = M<R>::sqrt(A * A + B * B);
+  // principal eigenvector: (A + Rm, B) for A >= 0, else (B, Rm - A); its
+  // squared norm is 2 Rm (Rm + |A|)
+  const R t = Rm + M<R>::fabs(A);
+  R c0 = R(1), s0 = R(0);
+  if (Rm > R(0)) {
+    const R nrm = M<R>::rsqrt(R(2) * Rm * t);
+    c0 = (A >= R(0) ? t : B) * nrm;
+    s0 = (A >= R(0) ? B : t) * nrm;
+  }
+  const R v = u01<R>(wv);
+  const bool lobe = v * Mm < Rm;
+  const R r = lobe ? M<R>::sqrt(qdiv(v * Mm, Rm)) : R(1);
+  R sa, ca;
+  M<R>::sincos_rev(static_cast<R>(static_cast<float>(wa >> 9) + 0.5f) * static_cast<R>(1.1920928955078125e-7), &sa, &ca);
+  const R px = (lobe ? R(1) : R(0)) + r * ca, py = r * sa;
+  R nrm = M<R>::rsqrt(px * px + py * py);
+  if (wa & 0x100u) nrm = -nrm;
+  const R cx = px * nrm, sx = py * nrm;
+  cph = c0 * cx - s0 * sx;
+  sph = s0 * cx + c0 * sx;
+  F = a * cph * cph + b * sph * sph + c * cph * sph;
 }
 
 // Inverse-CDF lookup: SamplingTable::Sample (src/table.cpp:50-66).  The
@@ -693,8 +737,8 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     const uint4 b = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), 0u);
     u_fp = u01<R>(b.x);
     ph.w_th = (b.y & 0xFFFFFF00u) | (b.x & 0xFFu);  // theta: high 24 bits; low byte: aux bits 16-23
-    ph.w_pa = (b.z & 0xFFFFF800u) | ((b.y & 0xFFu) << 3);  // trial-1 angle: high 21 bits; aux bits 8-15
-    ph.w_aa = (b.w & 0xFFFFF800u) | ((b.z & 0xFFu) << 3);  // trial-1 accept: high 21 bits; aux bits 0-7
+    ph.w_ph = b.z;                                   // azimuth component: bits 8-31; low byte: aux bits 0-7
+    ph.w_az = (b.w & 0xFFFFFF00u) | (b.y & 0xFFu);  // azimuth angle: bits 8-31; low byte: aux bits 8-15
   }
   R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
   const V3<R> dir = ph.P.s;
@@ -819,43 +863,29 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   const R a = s2sq * e1sq + s1sq * e2sq;
   const R b = s1sq * e1sq + s2sq * e2sq;
   const R c = R(2) * (s2sq - s1sq) * e12;
-  const R diff = a - b;
-  const R Fmax = R(0.5) * (a + b) + R(0.5) * M<R>::sqrt(diff * diff + c * c);
   R cph = R(1), sph = R(0), F = R(0);
   int trials = 0;
-  auto trial = [&](R pa, R aa) -> bool {
-    R sp_, cp_;
-    M<R>::sincos_rev(pa, &sp_, &cp_);
-    const R Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
-    ++trials;
-    if (aa * Fmax <= (Ft > R(0) ? Ft : R(0))) {
-      cph = cp_;
-      sph = sp_;
-      F = Ft;
-      return true;
-    }
-    return false;
-  };
   if constexpr (kXo) {
+    // the reference's rejection sampler in its draw order
+    const R diff = a - b;
+    const R Fmax = R(0.5) * (a + b) + R(0.5) * M<R>::sqrt(diff * diff + c * c);
     while (trials < 256) {
       const R pa = static_cast<R>(ph.xo.s.uniform());
       const R aa = static_cast<R>(ph.xo.s.uniform());
-      if (trial(pa, aa)) break;
-    }
-  } else {
-    // trial 1 from the event's main block, then three trials per extra block
-    // (21-bit uniforms: the angle's and the test's high 21 bits of a word; the
-    // third trial joins the four words' low 11 bits)
-    if (!trial(u01_21<R>(ph.w_pa >> 11), u01_21<R>(ph.w_aa >> 11))) {
-      for (uint32_t k = 1; trials < 256; ++k) {
-        const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), k);
-        if (trial(u01_21<R>(blk.x >> 11), u01_21<R>(blk.y >> 11))) break;
-        if (trial(u01_21<R>(blk.z >> 11), u01_21<R>(blk.w >> 11))) break;
-        if (trial(u01_21<R>(((blk.x & 0x7FFu) << 10) | ((blk.y & 0x7FFu) >> 1)),
-                  u01_21<R>(((blk.z & 0x7FFu) << 10) | ((blk.w & 0x7FFu) >> 1))))
-          break;
+      R sp_, cp_;
+      M<R>::sincos_rev(pa, &sp_, &cp_);
+      const R Ft = a * cp_ * cp_ + b * sp_ * sp_ + c * cp_ * sp_;
+      ++trials;
+      if (aa * Fmax <= (Ft > R(0) ? Ft : R(0))) {
+        cph = cp_;
+        sph = sp_;
+        F = Ft;
+        break;
       }
     }
+  } else {
+    azimuth_sample(a, b, c, ph.w_ph, ph.w_az, cph, sph, F);
+    trials = 1;
   }
   // frame update P <- A(theta, phi) P (simulation.cpp:438-453)
   const Frame<R> Po = ph.P;
@@ -919,10 +949,13 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     R ur;
     if constexpr (kXo) {
       ur = static_cast<R>(ph.xo.s.uniform());
-    } else if (Ld.sp_count > 1) {  // the auxiliary draw went to the species choice
-      const uint4 blk = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events - 1),
-                                    0xFFFFFFFEu);
-      ur = u01<R>(blk.x);
+    } else if (Ld.sp_count > 1) {
+      // the auxiliary draw chose the species; where it fell inside that
+      // species' CDF interval is uniform again (24 bits times its probability)
+      const R* c = v.sel + Ld.sel_off;
+      const int k = sp - Ld.sp_first;
+      const R lo = k > 0 ? c[k - 1] : R(0);
+      ur = qdiv(aux_uniform<R>(ph) - lo, c[k] - lo);
     } else {
       ur = aux_uniform<R>(ph);
     }
@@ -1289,6 +1322,19 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
+}
+
+template <class R>
+__global__ void k_azimuth_probe(int n, const double* abc, const uint32_t* words, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    R cph, sph, F;
+    azimuth_sample<R>(static_cast<R>(abc[3 * i]), static_cast<R>(abc[3 * i + 1]), static_cast<R>(abc[3 * i + 2]),
+                      words[2 * i], words[2 * i + 1], cph, sph, F);
+    out[3 * i] = static_cast<double>(cph);
+    out[3 * i + 1] = static_cast<double>(sph);
+    out[3 * i + 2] = static_cast<double>(F);
+  }
 }
 
 __global__ void k_philox_probe(int n, const uint4* ctr, uint32_t k0, uint32_t k1, uint4* out) {

@@ -1054,6 +1054,30 @@ int lmc_batch_zero(lmc_batch* b) {
   return zero_set(&b->set);
 }
 
+int lmc_probe_azimuth(int32_t n, int32_t precision, const double* abc, const uint32_t* words, double* out) {
+  if (n <= 0 || !abc || !words || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
+  if (precision != LMC_PREC_FP32 && precision != LMC_PREC_FP64) return fail(LMC_ERR_INVALID, "unknown precision");
+  int rc = ensure_device();
+  if (rc) return rc;
+  double *d_abc = nullptr, *d_out = nullptr;
+  uint32_t* d_w = nullptr;
+  LMC_HIP(hipMalloc(&d_abc, sizeof(double) * 3 * n));
+  LMC_HIP(hipMalloc(&d_out, sizeof(double) * 3 * n));
+  LMC_HIP(hipMalloc(&d_w, sizeof(uint32_t) * 2 * n));
+  LMC_HIP(hipMemcpy(d_abc, abc, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  LMC_HIP(hipMemcpy(d_w, words, sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice));
+  if (precision == LMC_PREC_FP32)
+    hipLaunchKernelGGL(k_azimuth_probe<float>, dim3((n + 255) / 256), dim3(256), 0, 0, n, d_abc, d_w, d_out);
+  else
+    hipLaunchKernelGGL(k_azimuth_probe<double>, dim3((n + 255) / 256), dim3(256), 0, 0, n, d_abc, d_w, d_out);
+  LMC_HIP(hipGetLastError());
+  LMC_HIP(hipMemcpy(out, d_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  LMC_HIP(hipFree(d_abc));
+  LMC_HIP(hipFree(d_out));
+  LMC_HIP(hipFree(d_w));
+  return LMC_OK;
+}
+
 int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
   if (n <= 0 || !ctr || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
   int rc = ensure_device();

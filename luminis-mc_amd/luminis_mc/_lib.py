@@ -156,6 +156,7 @@ def lib():
     L.lmc_mie_amplitudes.argtypes = [c_double, c_double, c_double, c_int32, c_dp, c_dp, c_dp]
     L.lmc_mie_info.argtypes = [c_double, c_double, c_double, c_dp, c_dp, c_dp]
     L.lmc_sampling_cdf.argtypes = [c_int32, c_double, c_double, c_dp, c_dp]
+    L.lmc_probe_azimuth.argtypes = [c_int32, c_int32, c_dp, ctypes.POINTER(ctypes.c_uint32), c_dp]
     L.lmc_probe_philox.argtypes = [c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     if L.lmc_abi_version() != 1:

@@ -205,3 +205,41 @@ def test_param_build_rewrites_coefficients():
     assert tr.base.layers[0].mu_s == 15.0  # the template is untouched
     with pytest.raises(ValueError):
         _ParamBuild(tr.base, np.array([[15.0, 0.01, 0.0], [8.0, 0.05, 1.5]]))
+
+
+def _azimuth_numpy(abc, words, mistake=None):
+    """Restatement of the device azimuth sampler with optional planted mistakes."""
+    a, b, c = abc
+    M, A, B = 0.5 * (a + b), 0.5 * (a - b), 0.5 * c
+    Rm = np.hypot(A, B)
+    t = Rm + abs(A)
+    c0, s0 = (1.0, 0.0) if Rm == 0 else ((t, B) if A >= 0 else (B, t)) / np.sqrt(2 * Rm * t)
+    if mistake == "axis":  # minor instead of principal axis
+        c0, s0 = -s0, c0
+    v = ((words[:, 0] >> 8).astype(np.float64) + 0.5) / 2.0 ** 24
+    lobe = v * M < (Rm if mistake != "weight" else Rm * Rm / M)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r2 = np.where(lobe, v * M / Rm, 1.0)
+    r = r2 if mistake == "radius" else np.sqrt(r2)
+    ang = 2 * np.pi * ((words[:, 1] >> 9).astype(np.float64) + 0.5) / 2.0 ** 23
+    px, py = np.where(lobe, 1.0, 0.0) + r * np.cos(ang), r * np.sin(ang)
+    nrm = 1 / np.hypot(px, py)
+    if mistake != "half":
+        nrm = np.where(words[:, 1] & 0x100, -nrm, nrm)
+    cx, sx = px * nrm, py * nrm
+    return c0 * cx - s0 * sx, s0 * cx + c0 * sx
+
+
+def test_azimuth_lattice_check_is_sensitive():
+    """The lattice CDF check applied to the device sampler passes the exact
+    mixture sampler and fails each plausible slip (wrong axis, wrong mixture
+    weight, radius not square-rooted, half period ignored)."""
+    import azimuth_check as ac
+    forms = ac.forms(np.random.default_rng(11), 12)
+    words = ac.lattice_words()
+    worst = max(ac.cdf_discrepancy(f, *_azimuth_numpy(f, words)) for f in forms)
+    assert worst < ac.CDF_TOL / 4
+    for mistake in ("axis", "weight", "radius", "half"):
+        bad = max(ac.cdf_discrepancy(f, *_azimuth_numpy(f, words, mistake)) for f in forms)
+        assert bad > 5 * ac.CDF_TOL, mistake
+
