@@ -32,10 +32,6 @@ template <> struct M<float> {
   }
   static __device__ __forceinline__ void sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
   static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
-  // x^(2/3) for x >= 0 through the hardware log2/exp2
-  static __device__ __forceinline__ float pow23(float x) {
-    return __builtin_amdgcn_exp2f(0.666666667f * __builtin_amdgcn_logf(x));
-  }
 };
 template <> struct M<double> {
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -49,7 +45,6 @@ template <> struct M<double> {
   }
   static __device__ __forceinline__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
   static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
-  static __device__ __forceinline__ double pow23(double x) { return ::cbrt(x * x); }
 };
 
 template <class R> struct cx {
