@@ -23,7 +23,22 @@ template <> struct M<float> {
   static __device__ __forceinline__ float log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
   static __device__ __forceinline__ float sqrt(float x) { return __fsqrt_rn(x); }
   static __device__ __forceinline__ float rsqrt(float x) { return __frsqrt_rn(x); }
-  static __device__ __forceinline__ float acos(float x) { return acosf(x); }
+  // acos by Abramowitz & Stegun 4.4.46 (Hastings): acos|x| = sqrt(1 - |x|) P7(|x|),
+  // |error| <= 2e-8 in exact arithmetic (4e-7 abs, 2e-7 rel near 0 in FP32);
+  // acos x = pi - acos|x| for x < 0.
+  static __device__ __forceinline__ float acos(float x) {
+    const float ax = fabsf(x);
+    float p = -0.0012624911f;
+    p = __builtin_fmaf(p, ax, 0.0066700901f);
+    p = __builtin_fmaf(p, ax, -0.0170881256f);
+    p = __builtin_fmaf(p, ax, 0.0308918810f);
+    p = __builtin_fmaf(p, ax, -0.0501743046f);
+    p = __builtin_fmaf(p, ax, 0.0889789874f);
+    p = __builtin_fmaf(p, ax, -0.2145988016f);
+    p = __builtin_fmaf(p, ax, 1.5707963050f);
+    const float r = __fsqrt_rn(1.0f - ax) * p;
+    return x < 0.0f ? 3.14159265358979f - r : r;
+  }
   static __device__ __forceinline__ float atan2(float y, float x) { return atan2f(y, x); }
   // sin/cos of 2*pi*u for u in [0,1): the hardware evaluates in revolutions.
   static __device__ __forceinline__ void sincos_rev(float u, float* s, float* c) {

@@ -243,7 +243,7 @@ __device__ __forceinline__ R table_sample(const SceneView<R>& v, const SpeciesDe
   if (lo == 0) return s.cdf_min;
   const R ca = c[lo - 1], cb = c[lo];
   const R xa = s.cdf_min + static_cast<R>(lo - 1) * s.cdf_step;
-  const R t = (u - ca) / (cb - ca);
+  const R t = qdiv(u - ca, cb - ca);
   return xa + t * s.cdf_step;
 }
 
@@ -418,6 +418,9 @@ template <class R> __device__ __forceinline__ R stokes(int q, cx<R> Ex, cx<R> Ey
 
 // Deposit the 8 CBS grids for one detection: coherent |E_f + E_r|^2 terms and
 // incoherent |E_f|^2 + |E_r|^2 terms (accumulate_stokes, detector.cpp:1194-1226).
+// A time-resolved hit goes to its time bin only; the reference's second copy
+// in the time-integrated bin 0 is the sum of bins 1.. and is formed when the
+// accumulators are read (fold_time_bins), which halves these atomics.
 template <class R, bool kAdj>
 __device__ __forceinline__ void cbs_deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int64_t bin,
                                             int t_idx, cx<R> Efx, cx<R> Efy, cx<R> Erx, cx<R> Ery, const R* score) {
@@ -427,9 +430,7 @@ __device__ __forceinline__ void cbs_deposit(const KParams<R>& p, const SensorDev
 #pragma unroll 1
   for (int q = 0; q < 8; ++q) {
     const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
-    const double dv = static_cast<double>(val);
-    deposit<R, kAdj>(p, sd, lds_grid, q * plane + bin, dv, score);
-    if (t_idx >= 1) deposit<R, kAdj>(p, sd, lds_grid, q * plane + t_idx * tplane + bin, dv, score);
+    deposit<R, kAdj>(p, sd, lds_grid, q * plane + t_idx * tplane + bin, static_cast<double>(val), score);
   }
 }
 
@@ -836,8 +837,8 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   const int kind = S.kind;
   if (kind == LMC_PHASE_HG) {  // phase.cpp:108-115
     const R g = S.hg_g, gg = g * g;
-    const R frac = (R(1) - gg) / (R(1) - g + R(2) * g * uth);
-    R c = (R(1) + gg - frac * frac) / (R(2) * g);
+    const R frac = qdiv(R(1) - gg, R(1) - g + R(2) * g * uth);
+    R c = qdiv(R(1) + gg - frac * frac, R(2) * g);
     if constexpr (sizeof(R) == 4) c = clamp1(c);
     theta = M<R>::acos(c);
     if constexpr (kAdj) {
@@ -1179,6 +1180,8 @@ __global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
 // propagation phase exp(i k0 optical_path) (:546-568), and the far-field
 // fluence and reflectance sensors.  Small grids accumulate in a per-workgroup
 // LDS copy (ds_add_f64), the rest straight to HBM (global_atomic_add_f64).
+// Time-resolved CBS and planar-fluence hits are deposited in their time bin
+// only; bin 0 (time-integrated) is completed at read-out.
 template <class R, bool kAdj>
 __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1261,9 +1264,8 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         const cx<R> Ex = cscale(Efx, R(-1));  // lab x negated (detector.cpp:812)
 #pragma unroll 1
         for (int q = 0; q < 4; ++q) {
-          const double dv = static_cast<double>(stokes(q, Ex, Efy));
-          if (t_idx >= 1) deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, dv, score);
-          deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, dv, score);
+          deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, static_cast<double>(stokes(q, Ex, Efy)),
+                           score);
         }
       } else if (sd.kind == LMC_SENSOR_PLANAR_FIELD) {
         double sp, cp;

@@ -896,6 +896,27 @@ int zero_set(lmc_scene* s) {
   return LMC_OK;
 }
 
+// The sensor kernel deposits a time-resolved CBS or planar-fluence hit in its
+// time bin t >= 1 only; the reference also adds it to the time-integrated bin
+// 0 (detector.cpp:1223-1225, :829-840).  Bin 0 of a read-out copy of scene k's
+// accumulators (`buf` holds n_out values from out_base on) becomes its own
+// steady-state deposits plus the sum of bins 1..n_t-1.
+void fold_time_bins(const lmc_scene* s, int k, double* buf) {
+  const SceneInfo& sc = s->scenes[k];
+  for (size_t j = 0; j < s->sensors_host.size(); ++j) {
+    if (s->sensor_scene[j] != k) continue;
+    const SensorDev& sd = s->sensors_host[j];
+    const int nq = sd.kind == LMC_SENSOR_FARFIELD_CBS ? 8 : (sd.kind == LMC_SENSOR_PLANAR_FLUENCE ? 4 : 0);
+    if (!nq || sd.n2 < 2) continue;
+    const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1, plane = tplane * sd.n2;
+    for (int q = 0; q < nq; ++q) {
+      double* g = buf + (sd.out_off - sc.out_base) + q * plane;
+      for (int t = 1; t < sd.n2; ++t)
+        for (int64_t i = 0; i < tplane; ++i) g[i] += g[t * tplane + i];
+    }
+  }
+}
+
 // Add scene k's accumulators and hit counters into caller buffers.
 int read_scene(lmc_scene* s, int k, double* out, int64_t n, uint64_t* hits) {
   const SceneInfo& sc = s->scenes[k];
@@ -904,6 +925,7 @@ int read_scene(lmc_scene* s, int k, double* out, int64_t n, uint64_t* hits) {
     std::vector<double> tmp(sc.n_out);
     if (sc.n_out)
       LMC_HIP(hipMemcpy(tmp.data(), s->out.p + sc.out_base, sizeof(double) * sc.n_out, hipMemcpyDeviceToHost));
+    fold_time_bins(s, k, tmp.data());
     for (int64_t i = 0; i < sc.n_out; ++i) out[i] += tmp[i];
   }
   if (hits) {
@@ -1004,6 +1026,7 @@ int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int
   if (grad) {
     std::vector<double> tmp(need);
     LMC_HIP(hipMemcpy(tmp.data(), s->grad.p, sizeof(double) * need, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 3 * nl; ++q) fold_time_bins(s, 0, tmp.data() + q * s->n_out);  // rows share the layout
     for (int64_t i = 0; i < need; ++i) grad[i] += tmp[i];
   }
   return LMC_OK;

@@ -80,6 +80,50 @@ def test_gpu_fp64_xoshiro_trajectory_parity_plain_mie(require_gpu):
     _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
 
 
+def _time_resolved_scene(n_photons, seed):
+    """Plain Mie medium with a time-resolved CBS sensor (16 x 4 angles, 8 time bins)
+    and a time-resolved planar fluence (16 x 16, 8 time bins) at z = 0."""
+    cfg, _ = scenes.config_plain_mie(n_photons, seed=seed)
+    len_t = 100.0 * 0.299792458 / 1.33
+    group = lmc.SensorsGroup()
+    group.add_detector(lmc.FarFieldCBSSensor(0.1, 2 * math.pi, len_t, 0.1 / 16, 2 * math.pi / 4, len_t / 8))
+    group.add_detector(lmc.PlanarFluenceSensor(0.0, 10.0, 10.0, len_t, 10.0 / 16, 10.0 / 16, len_t / 8,
+                                               absorb=False))
+    cfg.detector = group
+    return cfg
+
+
+def _bin0_minus_sum(block, n_q, n_t):
+    g = block.reshape(n_q, n_t, -1)
+    return np.abs(g[:, 0] - g[:, 1:].sum(axis=1)).max() / max(np.abs(g[:, 0]).max(), 1e-300)
+
+
+def test_gpu_fp64_xoshiro_parity_time_resolved(require_gpu):
+    """Time-resolved CBS and planar fluence: the device deposits each hit in its
+    time bin and completes the time-integrated bin 0 at read-out; the oracle
+    deposits twice as the reference does (detector.cpp:1223-1225, :829-840)."""
+    cfg = _time_resolved_scene(20000, 5)
+    sc = _scene(cfg)
+    st = sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
+    gpu, hits = sc.read()
+    cpu, chits, cev, _ = oracle.run(sc.build, 20000, 5)
+    print("events gpu/cpu", st["events"], cev, "hits", list(hits), list(chits))
+    (o_cbs, n_cbs), (o_flu, n_flu) = sc.build.offsets[:2]
+    n_t = n_cbs // (8 * 16 * 4)
+    assert n_t == n_flu // (4 * 16 * 16) and n_t > 2
+    for arr in (gpu, cpu):
+        assert _bin0_minus_sum(arr[o_cbs:o_cbs + n_cbs], 8, n_t) < 1e-9
+        assert _bin0_minus_sum(arr[o_flu:o_flu + n_flu], 4, n_t) < 1e-9
+    assert np.abs(gpu[o_cbs:o_cbs + n_cbs]).max() > 0 and np.abs(gpu[o_flu:o_flu + n_flu]).max() > 0
+    _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
+    # a second run accumulates: bin 0 of the read-out stays the sum of the time bins
+    sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000, off=20000))
+    gpu2, _ = sc.read()
+    assert _bin0_minus_sum(gpu2[o_cbs:o_cbs + n_cbs], 8, n_t) < 1e-9
+    assert _bin0_minus_sum(gpu2[o_flu:o_flu + n_flu], 4, n_t) < 1e-9
+    sc.close()
+
+
 def test_gpu_fp64_xoshiro_trajectory_parity_multilayer(require_gpu):
     cfg, d = scenes.config_multilayer(20000, seed=11, small=True, with_field=True)
     sc = _scene(cfg)
