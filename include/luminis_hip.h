@@ -161,7 +161,8 @@ typedef struct lmc_scene_desc {
 /* Precision / RNG modes. */
 enum { LMC_PREC_FP32 = 0, LMC_PREC_FP64 = 1 };
 enum {
-  LMC_RNG_PHILOX = 0,   /* Philox4x32-10 keyed by seed, counter by photon/event */
+  LMC_RNG_PHILOX = 0,   /* per-photon xoshiro128** streams seeded by Philox4x32-10
+                           of (seed, photon index); production path            */
   LMC_RNG_XOSHIRO = 1   /* xoshiro256++ re-seeded per photon with mix_seed, in
                            the reference's draw order (src/simulation.cpp:197);
                            FP64 only: trajectory-level comparison mode      */
@@ -242,8 +243,9 @@ int lmc_scene_zero(lmc_scene* s);
 /* ---- batched persistent launch over many independent scenes ----------- */
 /* No counterpart in the reference (which runs one SimConfig per call): up to
  * 4096 scenes, each validated as by lmc_scene_create, share one persistent
- * transport launch.  Scene i's photon j draws the same Philox (or xoshiro)
- * stream as photon j of lmc_scene_run on that scene alone with the same seed,
+ * transport launch.  Scene i's photon j draws the same random stream
+ * (production or xoshiro256++ check mode) as photon j of lmc_scene_run on
+ * that scene alone with the same seed,
  * so a batched scene reproduces its single-scene run (up to the summation
  * order of the FP64 accumulators). */
 typedef struct lmc_batch lmc_batch;
@@ -291,7 +293,7 @@ int lmc_sampling_cdf(int32_t n_div, double min_val, double max_val,
 int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
                      uint32_t* out);
 
-/* Azimuth sampler of the Philox path on device: for each i, the (cos phi,
+/* Azimuth sampler of the production path on device: for each i, the (cos phi,
  * sin phi, F(phi)) that the transport draws from the two 32-bit words
  * words[2i] (component and radius) and words[2i+1] (angle, half period) for
  * F(phi) = a cos^2 + b sin^2 + c sin cos, (a, b, c) = abc[3i..3i+2] (the

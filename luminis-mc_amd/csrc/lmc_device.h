@@ -115,6 +115,29 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
   }
   return c;
 }
+// xoshiro128** (Blackman & Vigna, "Scrambled linear pseudorandom number
+// generators", ACM TOMS 47, 2021): 128-bit state, 32-bit outputs, one cycle
+// of length 2^128 - 1 through every non-zero state.  Shifts, rotates and
+// xors only (the multiplications by 5 and 9 are shift-adds), so a 32-bit
+// output costs about nine full-rate vector instructions, where a Philox
+// block spends twenty 32x32->64 multiplies on 128 bits.
+struct Xoshiro128ss {
+  uint32_t s0, s1, s2, s3;
+  static __device__ __forceinline__ uint32_t rotl(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+  __device__ __forceinline__ uint32_t next() {
+    const uint32_t x5 = (s1 << 2) + s1;
+    const uint32_t r7 = rotl(x5, 7);
+    const uint32_t out = (r7 << 3) + r7;
+    const uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = rotl(s3, 11);
+    return out;
+  }
+};
 // 24-bit uniform on the open interval (0,1): never 0 (log) and never 1.
 template <class R> __device__ __forceinline__ R u01(uint32_t x) {
   return static_cast<R>((x >> 8) + 0.5f) * static_cast<R>(5.9604644775390625e-8);

@@ -132,7 +132,7 @@ template <class R, bool kXo> struct Photon {
   R w, gpl, t0;
   double opl;
   int32_t events, layer;
-  uint32_t idx;  // photon index, low word (the launch's high word is uniform)
+  Xoshiro128ss rng;  // this photon's event stream (seeded from Philox at launch)
   bool alive;
   // CBS history
   V3<R> m1, s1, mq, sq;  // frame after the first scatter, frame before the last one
@@ -141,8 +141,8 @@ template <class R, bool kXo> struct Photon {
   cx<R> b2, b1;          // last event's J_n = diag(b2, b1) R(bc, bs), pending
   R bc, bs;
   int32_t sp;            // (first species << 16) | last species
-  // Philox words of the current event kept past the flight (theta, the two
-  // azimuth draws, the shared 24-bit auxiliary draw)
+  // words of the current event kept past the flight (theta, the two azimuth
+  // draws, the shared 24-bit auxiliary draw)
   uint32_t w_th, w_ph, w_az;
   XoState<kXo> xo;
   R aL[kMaxGradLayers], aN[kMaxGradLayers], aG[kMaxGradLayers];
@@ -163,12 +163,17 @@ __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t
   return philox4x32_10(make_uint4(idx, idx_hi, ev, sub), k0v, k1v);
 }
 
-// Philox draw map of one event e (counter (photon, high word, e, 0)): word x
-// -> free path (high 24 bits), y -> theta (high 24), z -> azimuth component
-// and radius (high 24), w -> azimuth angle (high 23) and half period (bit 8);
-// the low bytes of x, y, z form one auxiliary 24-bit uniform (the species
-// choice in a mixture layer, otherwise the roulette draw).  One block per
-// event; emission draws use event 0xFFFFFFFF.
+// Random numbers of the production path.  Photon i of a run owns a stream:
+// Philox4x32-10 of the counter (i, high word of i, 0xFFFFFFFE, 0) under the
+// seed is the initial state of a xoshiro128** generator (a uniformly random
+// point of its 2^128 - 1 cycle).  The emission draws come first (four
+// outputs for an extended source, two for a pulse profile); then each event
+// takes the next four outputs as words x, y, z, w: x -> free path (high 24
+// bits), y -> theta (high 24), z -> azimuth component and radius (high 24),
+// w -> azimuth angle (high 23) and half period (bit 8); the low bytes of x,
+// y, z form one auxiliary 24-bit uniform (the species choice in a mixture
+// layer, otherwise the roulette draw).  Every draw is a function of the seed
+// and the photon index alone, whatever the launch shape.
 template <class R, bool kXo> __device__ __forceinline__ R aux_uniform(const Photon<R, kXo>& ph) {
   const uint32_t x = ((ph.w_th & 0xFFu) << 16) | ((ph.w_az & 0xFFu) << 8) | (ph.w_ph & 0xFFu);
   return static_cast<R>(static_cast<float>(x) + 0.5f) * static_cast<R>(5.9604644775390625e-8);
@@ -615,10 +620,13 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
 }
 
 // Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
-// The Philox stream draws emission uniforms from event slot 0xFFFFFFFF.
+// The production path draws the emission uniforms from the photon's stream.
 template <class R, bool kXo>
 __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>& L, Photon<R, kXo>& ph, uint32_t idx) {
-  ph.idx = idx;
+  if constexpr (!kXo) {
+    const uint4 sd = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFEu, 0u);
+    ph.rng = {sd.x | ((sd.x | sd.y | sd.z | sd.w) == 0u ? 1u : 0u), sd.y, sd.z, sd.w};  // never the zero state
+  }
   R x = static_cast<R>(L.lpos_x), y = static_cast<R>(L.lpos_y);
   R t0 = static_cast<R>(L.t_off * 1e-3);
   const R pulse = static_cast<R>(L.pulse * 1e-3);
@@ -649,7 +657,11 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
     t0 = static_cast<R>(td);
   } else {
     if (L.source != LMC_SOURCE_POINT) {
-      const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 0u);
+      uint4 b;
+      b.x = ph.rng.next();
+      b.y = ph.rng.next();
+      b.z = ph.rng.next();
+      b.w = ph.rng.next();
       const R sg = static_cast<R>(L.sigma);
       if (L.source == LMC_SOURCE_UNIFORM) {
         R sn, cs;
@@ -666,7 +678,9 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
       }
     }
     if (L.temporal != LMC_TIME_DELTA) {
-      const uint4 b = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFFu, 1u);
+      uint2 b;
+      b.x = ph.rng.next();
+      b.y = ph.rng.next();
       const R ua = u01<R>(b.x);
       switch (L.temporal) {
         case LMC_TIME_GAUSSIAN: {
@@ -735,7 +749,11 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   if constexpr (kXo) {
     u_fp = static_cast<R>(ph.xo.s.uniform());
   } else {
-    const uint4 b = event_block(p.key0, p.key1, ph.idx, p.idx_hi, static_cast<uint32_t>(ph.events), 0u);
+    uint4 b;
+    b.x = ph.rng.next();
+    b.y = ph.rng.next();
+    b.z = ph.rng.next();
+    b.w = ph.rng.next();
     u_fp = u01<R>(b.x);
     ph.w_th = (b.y & 0xFFFFFF00u) | (b.x & 0xFFu);  // theta: high 24 bits; low byte: aux bits 16-23
     ph.w_ph = b.z;                                   // azimuth component: bits 8-31; low byte: aux bits 0-7

@@ -19,7 +19,7 @@ row of a non-HG layer is zero.
 
 `DifferentiableTransport` exposes this as a torch.autograd.Function over a
 (n_layers, 3) tensor of (mu_s, mu_a, g) per layer; `crn_central_difference`
-is the finite-difference check (common random numbers: the same Philox
+is the finite-difference check (common random numbers: the same per-photon
 streams at theta +- h).
 """
 from __future__ import annotations
@@ -79,7 +79,7 @@ class DifferentiableTransport:
     the scene's flat accumulators (sensor after sensor, each in the layout of
     its grid) divided by n_photons; backward() applies the score gradients.
     Every call runs photons [photon_offset, photon_offset + n_photons) of the
-    same Philox streams (common random numbers across parameter values)."""
+    same per-photon random streams (common random numbers across parameter values)."""
 
     def __init__(self, sample: Sample, laser: Laser, sensors, n_photons: int, seed: int = 0,
                  max_events: int = 1000, track_reverse_paths: bool = True, precision: str = "fp32",

@@ -1,4 +1,4 @@
-"""Distribution check for the azimuth sampler of the Philox path.
+"""Distribution check for the azimuth sampler of the production path.
 
 The density is F(phi) = a cos^2 phi + b sin^2 phi + c sin phi cos phi
 (src/phase.cpp:27-49).  The sampler maps two words to phi; fed a regular

@@ -6,7 +6,7 @@ HG layer.  Sigma of both estimators from batch means over disjoint photon
 ranges; the two must agree within a few combined sigma.
 
 The gradients come from the production FP32 transport; the differences from
-the FP64 transport on the same Philox streams: a relative step of 1e-3 moves
+the FP64 transport on the same per-photon random streams: a relative step of 1e-3 moves
 1/mu_t and mu_s/mu_t by a few FP32 ulps only, so FP32 rounding of the
 perturbed coefficients would bias the difference quotient by several percent."""
 import numpy as np

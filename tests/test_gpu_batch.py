@@ -2,8 +2,8 @@
 
 * FP64 + xoshiro: every scene of a batch reproduces the oracle's run of that
   scene alone (trajectory-level parity, as for single scenes).
-* FP32 + Philox: a batched scene reproduces its single-scene run on the same
-  Philox streams.
+* FP32 production path: a batched scene reproduces its single-scene run on the same
+  per-photon random streams.
 * hipGraph replay: each replay adds exactly what an eager run adds, with
   other device work in between.
 """

@@ -5,9 +5,9 @@
   and draw order, so every accumulator must match the oracle to rounding
   (trajectory-level parity) -- plain Mie CBS and the three-media multilayer
   scene with fluence, far-field fluence, reflectance and field sensors.
-* FP32 + Philox (the production path): statistically identical to the oracle,
+* FP32 production path (Philox-seeded xoshiro128** streams): statistically identical to the oracle,
   per-bin z-scores from batch means (|z| < 5 everywhere, >= 98 % within 3 sigma),
-  and close to the FP64 build on the same Philox streams.
+  and close to the FP64 build on the same streams.
 """
 import ctypes
 import math
@@ -184,7 +184,7 @@ def test_gpu_fp32_close_to_fp64_on_same_streams(require_gpu):
     g64 = a64.reshape(8, -1)
     # same random numbers: only rounding-driven branch flips separate the two
     rel = np.abs(g32[[0, 4]] - g64[[0, 4]]).sum() / np.abs(g64[[0, 4]]).sum()
-    print("FP32 vs FP64 (same Philox streams) relative L1 difference of S0 coh/incoh:", rel)
+    print("FP32 vs FP64 (same streams) relative L1 difference of S0 coh/incoh:", rel)
     assert rel < 0.05
 
 

@@ -2,9 +2,9 @@
 
 * FP64 + xoshiro: trajectory-level agreement with the oracle (mixture species
   selection, 2048-node S tables, CBS and complex field accumulators).
-* FP32 + Philox: per-bin z-scores against the oracle from batch means.
-* A mixture of identical species is the homogeneous layer (same Philox
-  streams: the species draw is part of every event's block).
+* FP32 production path: per-bin z-scores against the oracle from batch means.
+* A mixture of identical species is the homogeneous layer (same random
+  streams: the species draw is part of every event's four words).
 """
 import math
 

@@ -1,7 +1,7 @@
-"""Device azimuth sampler of the Philox path against its exact distribution.
+"""Device azimuth sampler of the production path against its exact distribution.
 
 The reference draws phi with density F(phi) = a cos^2 phi + b sin^2 phi +
-c sin phi cos phi by rejection (src/phase.cpp:27-49); the Philox path samples
+c sin phi cos phi by rejection (src/phase.cpp:27-49); the production path samples
 the same density exactly from two words as a mixture of a uniform and a
 cos^2 lobe about the principal axis.  Checked on device: unit (cos, sin),
 F(phi) returned equal to the quadratic form, the half-period bit adding pi,
