@@ -293,6 +293,12 @@ int lmc_sampling_cdf(int32_t n_div, double min_val, double max_val,
 int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
                      uint32_t* out);
 
+/* Production random stream on device: the first n (<= 2^20) outputs of the
+ * xoshiro128** generator of photon `photon` under key (k0, k1) = (low, high
+ * word of the seed), seeded by Philox4x32-10 of (photon low, photon high,
+ * 0xFFFFFFFE, 0). */
+int lmc_probe_stream(uint32_t k0, uint32_t k1, uint64_t photon, int32_t n, uint32_t* out);
+
 /* Azimuth sampler of the production path on device: for each i, the (cos phi,
  * sin phi, F(phi)) that the transport draws from the two 32-bit words
  * words[2i] (component and radius) and words[2i+1] (angle, half period) for

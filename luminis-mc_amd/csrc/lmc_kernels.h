@@ -174,6 +174,11 @@ __device__ __forceinline__ uint4 event_block(const uint32_t key0, const uint32_t
 // y, z form one auxiliary 24-bit uniform (the species choice in a mixture
 // layer, otherwise the roulette draw).  Every draw is a function of the seed
 // and the photon index alone, whatever the launch shape.
+__device__ __forceinline__ Xoshiro128ss seed_stream(uint32_t key0, uint32_t key1, uint32_t idx, uint32_t idx_hi) {
+  const uint4 sd = event_block(key0, key1, idx, idx_hi, 0xFFFFFFFEu, 0u);
+  return {sd.x | ((sd.x | sd.y | sd.z | sd.w) == 0u ? 1u : 0u), sd.y, sd.z, sd.w};  // never the zero state
+}
+
 template <class R, bool kXo> __device__ __forceinline__ R aux_uniform(const Photon<R, kXo>& ph) {
   const uint32_t x = ((ph.w_th & 0xFFu) << 16) | ((ph.w_az & 0xFFu) << 8) | (ph.w_ph & 0xFFu);
   return static_cast<R>(static_cast<float>(x) + 0.5f) * static_cast<R>(5.9604644775390625e-8);
@@ -623,10 +628,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
 // The production path draws the emission uniforms from the photon's stream.
 template <class R, bool kXo>
 __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>& L, Photon<R, kXo>& ph, uint32_t idx) {
-  if constexpr (!kXo) {
-    const uint4 sd = event_block(p.key0, p.key1, idx, p.idx_hi, 0xFFFFFFFEu, 0u);
-    ph.rng = {sd.x | ((sd.x | sd.y | sd.z | sd.w) == 0u ? 1u : 0u), sd.y, sd.z, sd.w};  // never the zero state
-  }
+  if constexpr (!kXo) ph.rng = seed_stream(p.key0, p.key1, idx, p.idx_hi);
   R x = static_cast<R>(L.lpos_x), y = static_cast<R>(L.lpos_y);
   R t0 = static_cast<R>(L.t_off * 1e-3);
   const R pulse = static_cast<R>(L.pulse * 1e-3);
@@ -1354,6 +1356,14 @@ __global__ void k_azimuth_probe(int n, const double* abc, const uint32_t* words,
     out[3 * i] = static_cast<double>(cph);
     out[3 * i + 1] = static_cast<double>(sph);
     out[3 * i + 2] = static_cast<double>(F);
+  }
+}
+
+// First n outputs of photon `photon`'s stream under key (k0, k1), one lane.
+__global__ void k_stream_probe(uint32_t k0, uint32_t k1, uint64_t photon, int n, uint32_t* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Xoshiro128ss g = seed_stream(k0, k1, static_cast<uint32_t>(photon), static_cast<uint32_t>(photon >> 32));
+    for (int i = 0; i < n; ++i) out[i] = g.next();
   }
 }
 

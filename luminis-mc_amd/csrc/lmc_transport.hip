@@ -1117,4 +1117,17 @@ int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1, u
   return LMC_OK;
 }
 
+int lmc_probe_stream(uint32_t k0, uint32_t k1, uint64_t photon, int32_t n, uint32_t* out) {
+  if (n <= 0 || n > (1 << 20) || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
+  int rc = ensure_device();
+  if (rc) return rc;
+  uint32_t* dout = nullptr;
+  LMC_HIP(hipMalloc(&dout, sizeof(uint32_t) * n));
+  hipLaunchKernelGGL(k_stream_probe, dim3(1), dim3(64), 0, 0, k0, k1, photon, n, dout);
+  LMC_HIP(hipGetLastError());
+  LMC_HIP(hipMemcpy(out, dout, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  LMC_HIP(hipFree(dout));
+  return LMC_OK;
+}
+
 }  // extern "C"

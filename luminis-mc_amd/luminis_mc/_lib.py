@@ -159,6 +159,8 @@ def lib():
     L.lmc_probe_azimuth.argtypes = [c_int32, c_int32, c_dp, ctypes.POINTER(ctypes.c_uint32), c_dp]
     L.lmc_probe_philox.argtypes = [c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    L.lmc_probe_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_uint64, c_int32,
+                                   ctypes.POINTER(ctypes.c_uint32)]
     if L.lmc_abi_version() != 1:
         raise ImportError("libluminis_hip.so ABI version mismatch")
     _lib = L

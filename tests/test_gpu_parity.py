@@ -60,6 +60,43 @@ def test_gpu_philox_known_answers(require_gpu):
         assert list(out[i]) == oracle.philox(cs[i], 0xDEADBEEF, 0x12345678)
 
 
+def _device_stream(k0, k1, photon, n):
+    out = np.zeros(n, dtype=np.uint32)
+    _lib.check(_lib.lib().lmc_probe_stream(k0, k1, photon, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+               "lmc_probe_stream")
+    return out
+
+
+def test_gpu_stream_matches_restatement(require_gpu):
+    """A photon's production stream: Philox4x32-10 of (photon low, photon high,
+    0xFFFFFFFE, 0) under the seed as the xoshiro128** state, then its outputs;
+    the device against the host Philox and the host restatement."""
+    import rng_check
+    for (k0, k1) in ((0, 0), (0xDEADBEEF, 0x12345678)):
+        for photon in (0, 1, 12345, (1 << 32) + 7):
+            st = oracle.philox(np.array([photon & 0xFFFFFFFF, photon >> 32, 0xFFFFFFFE, 0], dtype=np.uint32), k0, k1)
+            if not any(st):
+                st[0] = 1
+            want = rng_check.xoshiro128ss(st, 64)
+            assert list(_device_stream(k0, k1, photon, 64)) == want, (k0, k1, photon)
+
+
+def test_gpu_stream_uniformity(require_gpu):
+    """2^20 outputs of one photon's stream: top bytes and 24-bit uniforms
+    uniform (chi-square over 256 bins within 5 sigma), each bit set half the
+    time, and no lag-1 correlation."""
+    x = _device_stream(0x9E3779B9, 7, 42, 1 << 20).astype(np.int64)
+    n = x.size
+    counts = np.bincount(x >> 24, minlength=256)
+    chi2 = ((counts - n / 256) ** 2 / (n / 256)).sum()
+    assert abs(chi2 - 255) < 5 * np.sqrt(2 * 255), chi2
+    bits = ((x[:, None] >> np.arange(32)) & 1).mean(axis=0)
+    assert np.all(np.abs(bits - 0.5) < 5 * 0.5 / np.sqrt(n)), bits
+    u = ((x >> 8) + 0.5) / 2 ** 24
+    r = np.corrcoef(u[:-1], u[1:])[0, 1]
+    assert abs(r) < 5 / np.sqrt(n), r
+
+
 def _compare_exact(gpu, cpu, sensors_offsets, rtol):
     for (off, n) in sensors_offsets:
         g, c = gpu[off:off + n], cpu[off:off + n]

@@ -217,3 +217,22 @@ def test_reverse_field_catches_a_wrong_interior_order():
     lab_bf /= np.linalg.norm(lab_bf)
     lab_or = Er[0] * frames[5][0] + Er[1] * frames[5][1]
     assert not np.allclose(lab_or, lab_bf, atol=1e-6)
+
+
+def test_stream_engine_has_full_period():
+    """The production generator's state update (xoshiro128**) has period
+    2^128 - 1: its minimal polynomial is primitive of degree 128.  The same
+    check fails for a wrong shift, a wrong rotation or reordered updates."""
+    import rng_check
+    assert rng_check.engine_period_is_maximal()
+    for slip in ("shift", "rotate", "order"):
+        assert not rng_check.engine_period_is_maximal(slip), slip
+
+
+def test_stream_first_outputs_by_hand():
+    """From state (1, 2, 3, 4): out = rotl(2 * 5, 7) * 9 = 1280 * 9 = 11520; the
+    update gives (7, 0, 1026, rotl(6, 11) = 12288), whose output is 0 (s1 = 0);
+    then (12295, 1029, 1029, rotl(12288, 11) = 25165824), output
+    rotl(5145, 7) * 9 = 658560 * 9 = 5927040."""
+    import rng_check
+    assert rng_check.xoshiro128ss((1, 2, 3, 4), 3) == [11520, 0, 5927040]
