@@ -1064,10 +1064,24 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     sPre = reinterpret_cast<uint64_t*>(carve(sizeof(uint64_t) * (p.n_scenes + 1)));
     for (int i = tid; i <= p.n_scenes; i += nt) sPre[i] = p.prefix[i];
     v.lc = p.lcs;
-    v.L = p.layers;
-    v.S = p.species;
     v.sens = p.sensors;
-    v.planes = p.planes;
+    if constexpr (kLdsTables) {
+      // a batch stages the per-event scene arrays of all its scenes (the
+      // phase and amplitude tables stay in HBM, read through the caches)
+      LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(carve(sizeof(LayerDev<R>) * p.n_layers));
+      SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(carve(sizeof(SpeciesDev<R>) * p.n_species));
+      PlaneDev* sPl = reinterpret_cast<PlaneDev*>(carve(sizeof(PlaneDev) * p.n_planes));
+      copy(sL, p.layers, sizeof(LayerDev<R>) * p.n_layers);
+      copy(sS, p.species, sizeof(SpeciesDev<R>) * p.n_species);
+      copy(sPl, p.planes, sizeof(PlaneDev) * p.n_planes);
+      v.L = sL;
+      v.S = sS;
+      v.planes = sPl;
+    } else {
+      v.L = p.layers;
+      v.S = p.species;
+      v.planes = p.planes;
+    }
   } else {
     LayerDev<R>* sL = reinterpret_cast<LayerDev<R>*>(smem + H::layers);
     SpeciesDev<R>* sS = reinterpret_cast<SpeciesDev<R>*>(smem + H::species);
@@ -1086,7 +1100,7 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     v.planes = sPl;
     q = smem + H::end;
   }
-  if constexpr (kLdsTables) {
+  if constexpr (kLdsTables && !kBatch) {
     R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));
     uint16_t* tg = reinterpret_cast<uint16_t*>(carve(sizeof(uint16_t) * p.guide_total + 4));
     STab<R>* ts = reinterpret_cast<STab<R>*>(carve(sizeof(STab<R>) * p.stab_total));
@@ -1187,9 +1201,9 @@ k_transport(KParams<R> p) {
   transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch>(p);
 }
 
-template <class R, bool kXo, bool kTrack, bool kAdj>
+template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch>
 __global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
-  transport_body<R, kXo, kTrack, true, kAdj, false>(p);
+  transport_body<R, kXo, kTrack, true, kAdj, kBatch>(p);
 }
 
 // Sensor kernel: one thread per queued crossing.  For every sensor that

@@ -495,22 +495,28 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   const size_t tab_lds = ((sizeof(R) * T.cdf_total + 15) & ~15ull) +
                          ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
                          ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
+  // A batch keeps the tables in HBM but can stage the per-event scene arrays
+  // (layers, species, planes of all its scenes) in LDS.
+  const size_t scene_lds = batch ? al16(sizeof(LayerDev<R>) * p.n_layers) + al16(sizeof(SpeciesDev<R>) * p.n_species) +
+                                       al16(sizeof(PlaneDev) * p.n_planes)
+                                 : 0;
   // Launch shape: 16 resident waves per CU (4 per SIMD at <= 128 VGPRs).  The
-  // tables go to LDS when two 512-thread workgroups per CU can hold them, or
-  // else (FP32 forward runs) when one 1024-thread workgroup can; otherwise
-  // 256-thread workgroups read them through the caches.
-  const size_t need = base_lds + tab_lds;
-  const bool fit512 = !batch && 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
-  const bool wide_ok = !batch && !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
+  // staged data (a single scene's tables, a batch's scene arrays) go to LDS
+  // when two 512-thread workgroups per CU can hold them, or else (FP32
+  // forward runs) when one 1024-thread workgroup can; otherwise 256-thread
+  // workgroups read them through the caches.
+  const size_t need = base_lds + (batch ? scene_lds : tab_lds);
+  const bool fit512 = 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
+  const bool wide_ok = !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
   int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : (wide_ok ? 1024 : 256));
   if (block % 64 != 0 || block > 1024) return fail(LMC_ERR_INVALID, "block_size must be a multiple of 64, <= 1024");
   if (block > 512 && !wide_ok)
     return fail(LMC_ERR_UNSUPPORTED, "1024-thread workgroups need FP32 forward runs with LDS-resident tables");
   const int bpc = r->blocks_per_cu > 0 ? r->blocks_per_cu : std::max(1, 1024 / block);
   const int grid = g_num_cu * bpc;
-  const bool lds_tables = block > 512 || (!batch && need * static_cast<size_t>(std::min(bpc, std::max(1, 1024 / block))) <=
-                                                         g_max_lds_cu &&
-                                          need <= g_max_lds_block);
+  const bool lds_tables =
+      block > 512 ||
+      (need * static_cast<size_t>(std::min(bpc, std::max(1, 1024 / block))) <= g_max_lds_cu && need <= g_max_lds_block);
   const size_t lds = lds_tables ? need : base_lds;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   const bool track = r->track_reverse_paths != 0;
@@ -532,11 +538,18 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   }
   p.q_count = s->qcount.p;
   void (*kt)(KParams<R>) = nullptr;
-  if (batch)
-    kt = track ? k_transport<R, kXo, true, false, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
-  else if (block > 512) {
-    if constexpr (sizeof(R) == 4 && !kAdj)
-      kt = track ? k_transport_wide<R, kXo, true, kAdj> : k_transport_wide<R, kXo, false, kAdj>;
+  if (block > 512) {
+    if constexpr (sizeof(R) == 4 && !kAdj) {
+      if (batch)
+        kt = track ? k_transport_wide<R, kXo, true, kAdj, true> : k_transport_wide<R, kXo, false, kAdj, true>;
+      else
+        kt = track ? k_transport_wide<R, kXo, true, kAdj, false> : k_transport_wide<R, kXo, false, kAdj, false>;
+    }
+  } else if (batch) {
+    if (track)
+      kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, true> : k_transport<R, kXo, true, false, kAdj, true>;
+    else
+      kt = lds_tables ? k_transport<R, kXo, false, true, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
   } else if (track)
     kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, false> : k_transport<R, kXo, true, false, kAdj, false>;
   else

@@ -22,7 +22,7 @@ from . import _lib
 # occupancy at <= 128 VGPRs; the candidates vary how they are grouped and how
 # often a wave returns to the global index counter.
 DEFAULT_CANDIDATES = (
-    (256, 4, 256), (512, 2, 256), (256, 4, 128), (256, 4, 512), (128, 8, 256), (256, 3, 256),
+    (256, 4, 256), (512, 2, 256), (1024, 1, 256), (256, 4, 128), (256, 4, 512), (128, 8, 256), (256, 3, 256),
 )
 
 _memo: dict = {}
@@ -77,16 +77,21 @@ def autotune(target, run: _lib.RunDesc, budgets=None, fraction: float = 0.05, mi
     for (block, bpc, claim) in candidates:
         r.block_size, r.blocks_per_cu, r.claim = int(block), int(bpc), int(claim)
         best = None
-        for _ in range(max(1, repeats)):
-            if batch:
-                st = target.run(trial_budgets, r, use_graph=False)
-            else:
-                r.n_photons = trial_n
-                st = target.run(r)
-            ms = st["kernel_ms"]
-            best = ms if best is None else min(best, ms)
+        try:
+            for _ in range(max(1, repeats)):
+                if batch:
+                    st = target.run(trial_budgets, r, use_graph=False)
+                else:
+                    r.n_photons = trial_n
+                    st = target.run(r)
+                ms = st["kernel_ms"]
+                best = ms if best is None else min(best, ms)
+        except NotImplementedError:  # a shape this run cannot use (e.g. 1024 threads for FP64)
+            continue
         table.append({"block_size": block, "blocks_per_cu": bpc, "claim": claim, "ms": round(best, 3)})
     target.zero()
+    if not table:
+        raise ValueError("no candidate launch shape is usable for this run")
     win = min(table, key=lambda t: t["ms"])
     res = dict(win)
     res["table"] = table
