@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LMC_ABI_VERSION 1
+#define LMC_ABI_VERSION 2
 
 /* Status codes (reference: std::invalid_argument -> ValueError,
  * src/simulation.cpp:95-103; fatal exits become LMC_ERR_*). */
@@ -128,6 +128,17 @@ typedef struct lmc_sensor_desc {
   int32_t n[4];
   int64_t out_offset;   /* offset (doubles) of this sensor's accumulators     */
   int64_t out_len;      /* number of doubles                                  */
+  /* Estimator mode (Sensor::estimator_enabled, detector.cpp:314-331): the
+   * sensor takes no crossings (detector.cpp:246-253); a FARFIELD_CBS sensor
+   * instead runs process_estimation at every scattering vertex
+   * (detector.cpp:1315-1566), every other kind records nothing
+   * (detector.cpp:419-422, 571-575, 844-846). */
+  int32_t estimator;
+  int32_t n_phi_explicit; /* > 0: the FARFIELD_CBS phi columns are these
+                             angles at tables[phi_offset] (set_phi_slices,
+                             detector.cpp:909-943); 0: (j + 1/2) * d_phi    */
+  int64_t phi_offset;
+  double theta_pp_max;  /* estimator theta cap when > 0 (detector.cpp:1382)   */
 } lmc_sensor_desc;
 
 /* Laser (include/luminis/core/laser.hpp:45-90, src/laser.cpp:29-99). */

@@ -47,6 +47,7 @@ template <> struct M<float> {
   }
   static __device__ __forceinline__ void sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
   static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
+  static __device__ __forceinline__ float exp(float x) { return __expf(x); }
 };
 template <> struct M<double> {
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -60,6 +61,7 @@ template <> struct M<double> {
   }
   static __device__ __forceinline__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
   static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
 };
 
 template <class R> struct cx {
@@ -272,6 +274,8 @@ template <class R> struct SpeciesDev {
   int32_t s_off;     // offset in the S pool (one STab entry per node)
   int32_t pad;
   R cdf_min, cdf_step, hg_g, inv_ds;
+  R pi_inorm;  // pi * I_norm, the estimator's angular normalisation (detector.cpp:105-118, 1436)
+  R alb;       // mu_s / (mu_a + mu_s) of the species (estimator weight, detector.cpp:1466)
 };
 
 // S1/S2 at one node: (s1.re, s1.im, s2.re, s2.im).
@@ -314,6 +318,11 @@ struct SensorDev {
   double g[8];
   int64_t out_off;
   int32_t lds_off, pad2;
+  // estimator mode: est != 0 takes no crossings; a FARFIELD_CBS sensor then
+  // deposits every vertex's forced scatter into its first est_imax theta rows
+  // (bin geometry at est_geom in the geometry pool), in the transport
+  // kernel's LDS copy at est_lds (doubles) or, when est_lds < 0, in HBM
+  int32_t est, est_imax, est_geom, est_lds;
   // conservative grid bounds for the transport kernel's queueing test: exit
   // cosine -s_z >= q_cmin, hit point in [q_x0, q_x1] x [q_y0, q_y1], arrival
   // in [q_t0, q_t1] (slightly wider than the grid; the sensor kernel bins exactly)

@@ -60,6 +60,8 @@ template <class R> struct KParams {
   const PlaneDev* planes;
   const LaunchConst<R>* lcs;     // one per scene
   const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
+  const R* est_geom;             // estimator bin geometry pool
+  int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
 };
 
 // One queued plane crossing accepted by at least one sensor of its plane:
@@ -302,12 +304,11 @@ template <class R> __device__ __forceinline__ R exit_phi(V3<R> s) {
 // reverse_field_prefix/suffix (detector.cpp:1080-1161).  P0/E0 are the
 // launch frame and Jones vector; Pq is the frame before the last scatter
 // (P_{n-1}), P1 the frame after the first scatter, Pn the exit frame and T the
-// raw interior product J_{n-1}...J_2.  Returns the normalised reverse field in
-// the exit basis (m_n, n_n).
+// raw interior product J_{n-1}...J_2.  The prefix (stages A and B, scatter at
+// r_n and the interior by reciprocity) does not depend on the exit frame.
 template <class R>
-__device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const STab<R>* stab, const SpeciesDev<R>& spn,
-                                              const SpeciesDev<R>& sp1, const Frame<R>& P1, const Frame<R>& Pq,
-                                              const Frame<R>& Pn, const cx<R> T[4], cx<R>& Erm, cx<R>& Ern) {
+__device__ __forceinline__ void reverse_prefix(const LaunchConst<R>& lc, const STab<R>* stab, const SpeciesDev<R>& spn,
+                                               const Frame<R>& Pq, const cx<R> T[4], cx<R>& xm, cx<R>& xn) {
   const V3<R> s0{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
   const V3<R> m0{lc.lm[0], lc.lm[1], lc.lm[2]};
   const V3<R> n0{lc.ln[0], lc.ln[1], lc.ln[2]};
@@ -322,8 +323,8 @@ __device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const ST
   const cx<R> E0m{lc.pol[0], lc.pol[1]}, E0n{lc.pol[2], lc.pol[3]};
   // E = S_a * rot2(mp_in, np, m0, n0) * E0
   R r00 = dot(mp_in, m0), r01 = dot(mp_in, n0), r10 = dot(np, m0), r11 = dot(np, n0);
-  cx<R> xm = cadd(cscale(E0m, r00), cscale(E0n, r01));
-  cx<R> xn = cadd(cscale(E0m, r10), cscale(E0n, r11));
+  xm = cadd(cscale(E0m, r00), cscale(E0n, r01));
+  xn = cadd(cscale(E0m, r10), cscale(E0n, r11));
   xm = cmul(a2, xm);
   xn = cmul(a1, xn);
   // rot2(m_{n-1}, -n_{n-1}, mp_out, np)
@@ -337,6 +338,13 @@ __device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const ST
   // Stage B: Q T^T Q with Q = diag(1,-1): [[T00, -T10], [-T01, T11]]
   xm = cadd(cmul(T[0], ym), cscale(cmul(T[2], yn), R(-1)));
   xn = cadd(cscale(cmul(T[1], ym), R(-1)), cmul(T[3], yn));
+}
+
+// Stage C of the reverse field and its normalisation (reverse_field_suffix,
+// detector.cpp:1123-1161), from the prefix (xm, xn).
+template <class R>
+__device__ __forceinline__ void reverse_suffix(const STab<R>* stab, const SpeciesDev<R>& sp1, const Frame<R>& P1,
+                                               const Frame<R>& Pn, cx<R> xm, cx<R> xn, cx<R>& Erm, cx<R>& Ern) {
   // Stage C: scatter at r_1, -s1 -> s_n
   const V3<R> s_in_c{-P1.s.x, -P1.s.y, -P1.s.z};
   const R th_c = M<R>::acos(clamp1(dot(s_in_c, Pn.s)));
@@ -346,12 +354,9 @@ __device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const ST
   const V3<R> npp = safe_unit(cross_unit(s_in_c, Pn.s), n1neg);
   const V3<R> mpp_in = safe_unit(cross_unit(npp, s_in_c), P1.m);
   const V3<R> mpp_out = safe_unit(cross_unit(npp, Pn.s), Pn.m);
-  r00 = dot(mpp_in, P1.m);
-  r01 = dot(mpp_in, n1neg);
-  r10 = dot(npp, P1.m);
-  r11 = dot(npp, n1neg);
-  ym = cadd(cscale(xm, r00), cscale(xn, r01));
-  yn = cadd(cscale(xm, r10), cscale(xn, r11));
+  R r00 = dot(mpp_in, P1.m), r01 = dot(mpp_in, n1neg), r10 = dot(npp, P1.m), r11 = dot(npp, n1neg);
+  cx<R> ym = cadd(cscale(xm, r00), cscale(xn, r01));
+  cx<R> yn = cadd(cscale(xm, r10), cscale(xn, r11));
   ym = cmul(c2, ym);
   yn = cmul(c1, yn);
   r00 = dot(Pn.m, mpp_out);
@@ -369,6 +374,16 @@ __device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const ST
     Erm = {R(0), R(0)};
     Ern = {R(0), R(0)};
   }
+}
+
+// reverse_field (detector.cpp:1165-1173) of a detected photon.
+template <class R>
+__device__ __forceinline__ void reverse_field(const LaunchConst<R>& lc, const STab<R>* stab, const SpeciesDev<R>& spn,
+                                              const SpeciesDev<R>& sp1, const Frame<R>& P1, const Frame<R>& Pq,
+                                              const Frame<R>& Pn, const cx<R> T[4], cx<R>& Erm, cx<R>& Ern) {
+  cx<R> xm, xn;
+  reverse_prefix(lc, stab, spn, Pq, T, xm, xn);
+  reverse_suffix(stab, sp1, P1, Pn, xm, xn, Erm, Ern);
 }
 
 // Accumulate one value: LDS copy (ds_add_f64) for privatised grids, HBM
@@ -477,13 +492,13 @@ __device__ __forceinline__ bool accepts(const SensorDev& sd, V3<R> hp, V3<R> dir
 // process_hit: time bin (detector.cpp:787-797, :1182-1190), exit angles
 // (:1274-1281), pixel (:546-556, :800-806).  Returns false when the crossing
 // falls outside the grid, where the reference deposits nothing and counts no
-// hit.  t_idx: time bin >= 1, 0 for a CBS sensor without a time axis, -1 for
-// the other kinds without one.
+// hit.  t_idx: time bin >= 1 of a time-resolved grid, else 0 (the grid row a
+// deposit goes to; row 0 of a time-resolved grid is completed at read-out).
 template <class R>
 __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R arr, V3<R> dir, int64_t& bin,
                                            int& t_idx) {
   const int kind = sd.kind;
-  t_idx = kind == LMC_SENSOR_FARFIELD_CBS ? 0 : -1;
+  t_idx = 0;  // time bin; 0 = steady state (the only row when dt == 0)
   bin = 0;
   if ((kind == LMC_SENSOR_FARFIELD_CBS || kind == LMC_SENSOR_PLANAR_FLUENCE) && sd.g[5] > 0) {
     const double a = static_cast<double>(arr);
@@ -624,6 +639,180 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   return killed;
 }
 
+// optical_depth_to_detector (detector.cpp:53-101) for a ray leaving the
+// vertex toward the detector plane above it: sum of mu_t over the layers it
+// crosses, from the vertex's layer upward.
+template <class R>
+__device__ __forceinline__ R est_optical_depth(const SceneView<R>& v, int n_layers, int layer, V3<R> p, V3<R> dir,
+                                               R L_total) {
+  if (n_layers == 1) return v.L[layer].mu_t * L_total;
+  R tau = R(0), traveled = R(0);
+  const R dz = dir.z;
+  const R eps = static_cast<R>(1e-15);
+  for (int guard = 0; guard < 2 * kMaxLayers && traveled < L_total - eps; ++guard) {
+    const R remaining = L_total - traveled;
+    R seg = remaining;
+    if (M<R>::fabs(dz) > eps) {
+      // next interface between p.z and p.z + dz * remaining: the layer's top
+      // face going up, its bottom face going down
+      const R zt = p.z + dz * remaining;
+      const bool up = dz < R(0);
+      const bool has = up ? (layer > 0 && v.L[layer].z_min >= zt) : (v.L[layer].nbr & 1) && v.L[layer].z_max <= zt;
+      if (has) {
+        const R sd = ((up ? v.L[layer].z_min : v.L[layer].z_max) - p.z) / dz;
+        if (sd > eps && sd < remaining) seg = sd;
+      }
+    }
+    tau += v.L[layer].mu_t * seg;
+    p.x += dir.x * seg;
+    p.y += dir.y * seg;
+    p.z += dir.z * seg;
+    traveled += seg;
+    if (seg >= remaining - eps) break;
+    const int next = dz > R(0) ? layer + 1 : layer - 1;
+    if (next < 0 || next >= n_layers) break;
+    layer = next;
+  }
+  return tau;
+}
+
+// FarFieldCBSSensor::process_estimation (detector.cpp:1315-1566) at one
+// vertex: the photon's next scatter is forced into every (theta, phi) bin of
+// the sensor, with the weight w * albedo * exp(-tau) * F / (pi I_norm) * dOmega
+// that lands there, the forward field of that scatter and, from the second
+// order on, its reverse-path partner; the reverse prefix (stages A, B) is
+// computed once per vertex.  The common phase exp(i k L) of the forward and
+// reverse fields cancels in every Stokes term and is not formed.  The lanes of
+// a wave start at different bins, so their LDS adds rarely meet.
+template <class R, bool kXo>
+__device__ void estimate_vertex(const KParams<R>& p, const SceneView<R>& v, const Photon<R, kXo>& ph, int si,
+                                const SpeciesDev<R>& S, R alb_layer) {
+  const SensorDev& sd = v.sens[si];
+  if (sd.f_ev_en && (ph.events < sd.f_ev_min || ph.events > sd.f_ev_max)) return;
+  const R tiny = static_cast<R>(sizeof(R) == 8 ? 1e-300 : 0.0);
+  if (!(S.pi_inorm > R(0))) return;
+  const bool single = ph.events == 0;
+  const R w_sc = ph.w * (single ? alb_layer : S.alb);
+  if (w_sc < tiny) return;
+  const LaunchConst<R>& lc = *v.lc;
+  const V3<R> e1{lc.lm[0], lc.lm[1], lc.lm[2]}, e2{lc.ln[0], lc.ln[1], lc.ln[2]};
+  const V3<R> s_in{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
+  const Frame<R>& Pc = ph.P;
+  cx<R> pm{R(0), R(0)}, pn{R(0), R(0)};
+  Frame<R> P1;
+  if (!single) {
+    cx<R> Tm[4] = {ph.T[0], ph.T[1], ph.T[2], ph.T[3]};
+    if (ph.events >= 2) {  // interior product of the estimated path: J_k ... J_2
+      const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
+      const cx<R> r1 = cadd(cscale(ph.T[1], ph.bc), cscale(ph.T[3], ph.bs));
+      const cx<R> r2 = cadd(cscale(ph.T[0], -ph.bs), cscale(ph.T[2], ph.bc));
+      const cx<R> r3 = cadd(cscale(ph.T[1], -ph.bs), cscale(ph.T[3], ph.bc));
+      Tm[0] = cmul(ph.b2, r0);
+      Tm[1] = cmul(ph.b2, r1);
+      Tm[2] = cmul(ph.b1, r2);
+      Tm[3] = cmul(ph.b1, r3);
+    }
+    reverse_prefix(lc, v.stab, S, Pc, Tm, pm, pn);
+    P1 = frame_of(ph.m1, ph.s1);
+  }
+  const SpeciesDev<R>& S1 = v.S[ph.sp >> 16];
+  const int Nth = sd.n0, Nph = sd.n1;
+  const R* gc = p.est_geom + sd.est_geom;  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each)
+  const R dphi = static_cast<R>(sd.g[4]);
+  const R tmax = static_cast<R>(sd.g[2]), dt = static_cast<R>(sd.g[5]);
+  const R zp = static_cast<R>(v.planes[sd.plane].z);
+  const int64_t tplane = static_cast<int64_t>(Nth) * Nph, plane = tplane * sd.n2;
+  const bool in_lds = sd.est_lds >= 0;
+  double* lg = in_lds ? v.lgrid + sd.est_lds : nullptr;
+  double* hg = p.out + sd.out_off;
+  const int nb = sd.est_imax * Nph;
+  if (nb <= 0) return;
+  int b = static_cast<int>(__lane_id()) % nb;
+  for (int k = 0; k < nb; ++k, b = (b + 1 == nb) ? 0 : b + 1) {
+    const int it = b / Nph, jp = b - it * Nph;
+    const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
+    const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
+    const R a1 = s_th * c_ph, a2 = s_th * s_ph;
+    const V3<R> so{e1.x * a1 + e2.x * a2 - s_in.x * c_th, e1.y * a1 + e2.y * a2 - s_in.y * c_th,
+                   e1.z * a1 + e2.z * a2 - s_in.z * c_th};
+    // intersect_plane (detector.cpp:36-47) with the sensor plane, normal +z
+    if (M<R>::fabs(so.z) < static_cast<R>(1e-14)) continue;
+    const R L = (zp - ph.pos.z) / so.z;
+    if (!(L > R(0))) continue;
+    const R Tr = M<R>::exp(-est_optical_depth(v, p.n_layers, ph.layer, ph.pos, so, L));
+    if (Tr < static_cast<R>(1e-20)) continue;
+    const V3<R> pv = cross_unit(Pc.s, so);
+    const R pnrm = M<R>::sqrt(dot(pv, pv));
+    if (pnrm < static_cast<R>(1e-12)) continue;
+    const V3<R> ph_hat = vscale(pv, R(1) / pnrm);
+    const R sin_phi = -dot(Pc.m, ph_hat), cos_phi = dot(Pc.n, ph_hat);
+    const R th = M<R>::acos(clamp1(dot(Pc.s, so)));
+    cx<R> s1, s2;
+    s_lookup(v.stab, S, th, s1, s2);
+    // phase_F (detector.cpp:1012-1023)
+    const R s22 = cnorm(s2), s11 = cnorm(s1), Emm = cnorm(ph.Em), Enn = cnorm(ph.En);
+    const R cc = cos_phi * cos_phi, ss = sin_phi * sin_phi;
+    const R F = Emm * (s22 * cc + s11 * ss) + Enn * (s22 * ss + s11 * cc) +
+                R(2) * (ph.Em.r * ph.En.r + ph.Em.i * ph.En.i) * (s22 - s11) * sin_phi * cos_phi;
+    if (F < tiny) continue;
+    const R w_bin = w_sc * Tr * ((F / S.pi_inorm) * (dom * dphi));
+    if (w_bin < tiny) continue;
+    int t_idx = 0;  // time_bin (detector.cpp:1182-1190); photon speed 1 in path units
+    if (dt > R(0)) {
+      const R arr = ph.t0 + (ph.gpl + L);
+      if (arr < R(0) || arr >= tmax) continue;
+      t_idx = static_cast<int>(arr / dt) + 1;
+      if (t_idx >= sd.n2) continue;
+    }
+    const R amp = M<R>::sqrt(w_bin);
+    // exit frame (scatter_frame, detector.cpp:1027-1044) and the normalised
+    // forward field (apply_scatter_normalized, :121-161)
+    R st, ct;
+    M<R>::sincos(th, &st, &ct);
+    Frame<R> Pe;
+    {
+      const R ctc = ct * cos_phi, cts = ct * sin_phi, stc = st * cos_phi, sts = st * sin_phi;
+      Pe.m = {ctc * Pc.m.x + cts * Pc.n.x - st * Pc.s.x, ctc * Pc.m.y + cts * Pc.n.y - st * Pc.s.y,
+              ctc * Pc.m.z + cts * Pc.n.z - st * Pc.s.z};
+      Pe.n = {-sin_phi * Pc.m.x + cos_phi * Pc.n.x, -sin_phi * Pc.m.y + cos_phi * Pc.n.y,
+              -sin_phi * Pc.m.z + cos_phi * Pc.n.z};
+      Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
+              stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
+    }
+    const R fi = amp / M<R>::sqrt(F);
+    const cx<R> um = cadd(cscale(ph.Em, cos_phi), cscale(ph.En, sin_phi));
+    const cx<R> un = cadd(cscale(ph.Em, -sin_phi), cscale(ph.En, cos_phi));
+    const cx<R> fm = cscale(cmul(s2, um), fi), fn = cscale(cmul(s1, un), fi);
+    const cx<R> Efx = cadd(cscale(fm, Pe.m.x), cscale(fn, Pe.n.x));
+    const cx<R> Efy = cadd(cscale(fm, Pe.m.y), cscale(fn, Pe.n.y));
+    cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
+    if (!single) {
+      cx<R> rm, rn;
+      reverse_suffix(v.stab, S1, P1, Pe, pm, pn, rm, rn);
+      // CBS path phase exp(i k (s_out + s_in) . (r_vertex - r_1))
+      const R arg = lc.k0 * ((so.x + s_in.x) * (ph.pos.x - ph.r1.x) + (so.y + s_in.y) * (ph.pos.y - ph.r1.y) +
+                             (so.z + s_in.z) * (ph.pos.z - ph.r1.z));
+      R sa, ca;
+      M<R>::sincos(arg, &sa, &ca);
+      const cx<R> pp{ca * amp, sa * amp};
+      rm = cmul(rm, pp);
+      rn = cmul(rn, pp);
+      Erx = cadd(cscale(rm, Pe.m.x), cscale(rn, Pe.n.x));
+      Ery = cadd(cscale(rm, Pe.m.y), cscale(rn, Pe.n.y));
+    }
+    const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
+    const int64_t off = t_idx * tplane + b;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
+      acc_add(lg, hg, in_lds, q * plane + off, static_cast<double>(val));
+    }
+  }
+  const uint64_t done = __ballot(1);  // hits += 1 per estimated vertex
+  if (__lane_id() == __ffsll(static_cast<unsigned long long>(done)) - 1)
+    atomicAdd(&v.lhits[si], static_cast<unsigned>(__popcll(done)));
+}
+
 // Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
 // The production path draws the emission uniforms from the photon's stream.
 template <class R, bool kXo>
@@ -745,7 +934,7 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
-template <class R, bool kXo, bool kTrack, bool kAdj>
+template <class R, bool kXo, bool kTrack, bool kAdj, bool kEst>
 __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
   R u_fp;
   if constexpr (kXo) {
@@ -847,6 +1036,10 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     sp += lo < Ld.sp_count ? lo : Ld.sp_count - 1;
   }
   const SpeciesDev<R>& S = v.S[sp];
+  if constexpr (kEst) {  // SensorsGroup::run_estimators (detector.cpp:272-281)
+    for (int si = 0; si < p.n_sensors; ++si)
+      if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_vertex<R, kXo>(p, v, ph, si, S, Ld.albedo);
+  }
   // theta (medium.cpp:51-58 -> phase.cpp)
   R uth;
   if constexpr (kXo)
@@ -1028,7 +1221,7 @@ template <class R> struct LdsHdr {
 // The body is shared by two entry points: 512-thread workgroups (two per CU
 // when their LDS allows) and 1024-thread workgroups (one per CU, the whole
 // LDS for tables that two workgroups could not hold).
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false>
 __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
@@ -1119,6 +1312,11 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     v.stab = p.stab;
     v.sel = p.sel;
   }
+  if constexpr (kEst) {  // the workgroup's copy of the estimator grids that fit in LDS
+    double* eg = reinterpret_cast<double*>(carve(sizeof(double) * p.est_lds_n));
+    for (int i = tid; i < p.est_lds_n; i += nt) eg[i] = 0.0;
+    v.lgrid = eg;
+  }
   __syncthreads();
 
   // persistent photon loop with wave-level refill from a claimed index pool
@@ -1176,7 +1374,7 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     }
     int32_t ev0 = ph.events;
     if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj>(p, v, ph);
+      const int tr = transport_event<R, kXo, kTrack, kAdj, kEst>(p, v, ph);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));
@@ -1190,6 +1388,19 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   __syncthreads();
   if (tid == 0) atomicAdd(&p.stats[3], static_cast<unsigned long long>(sHits[kMaxSensors]));
   if (lane == 0) atomicAdd(&p.stats[1], static_cast<unsigned long long>(w_ev));
+  if constexpr (kEst) {  // estimator hits and the LDS grid copies go to HBM
+    for (int si = 0; si < p.n_sensors; ++si) {
+      const SensorDev& sd = v.sens[si];
+      if (!sd.est) continue;
+      if (tid == 0 && sHits[si]) atomicAdd(&p.hits[si], static_cast<unsigned long long>(sHits[si]));
+      if (sd.kind != LMC_SENSOR_FARFIELD_CBS || sd.est_lds < 0) continue;
+      const int64_t len = 8ll * sd.n0 * sd.n1 * sd.n2;
+      for (int64_t i = tid; i < len; i += nt) {
+        const double val = v.lgrid[sd.est_lds + i];
+        if (val != 0.0) unsafeAtomicAdd(p.out + sd.out_off + i, val);
+      }
+    }
+  }
 }
 
 // FP32 forward runs: at most 128 VGPRs, so 16 waves of a CU stay resident
@@ -1204,6 +1415,15 @@ k_transport(KParams<R> p) {
 template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch>
 __global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
   transport_body<R, kXo, kTrack, true, kAdj, kBatch>(p);
+}
+
+// Estimator runs (a FARFIELD_CBS sensor in estimator mode): the forced-scatter
+// loop over the sensor's bins dominates, so 256-thread workgroups and up to
+// 168 VGPRs (3 waves per SIMD); tables in LDS when they fit beside the grid.
+template <class R, bool kXo, bool kLdsTables>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2)))
+k_transport_est(KParams<R> p) {
+  transport_body<R, kXo, true, kLdsTables, false, false, true>(p);
 }
 
 // Sensor kernel: one thread per queued crossing.  For every sensor that

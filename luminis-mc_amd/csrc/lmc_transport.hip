@@ -107,6 +107,7 @@ template <class R> struct DevTables {
   DBuf<uint16_t> guide;
   DBuf<STab<R>> stab;
   DBuf<LaunchConst<R>> lcs;
+  DBuf<R> est_geom;
   int cdf_total = 0, guide_total = 0, stab_total = 0, sel_total = 0;
   size_t table_bytes = 0;
 };
@@ -146,6 +147,11 @@ struct lmc_scene {
   std::vector<int> sensor_scene;
   std::vector<PlaneDev> planes_host;
   int lds_grid = 0;
+  // estimator-mode CBS sensors: bin geometry (double, converted per precision)
+  // and the doubles of their grids staged in the transport kernel's LDS
+  std::vector<double> est_geom_host;
+  int est_lds_n = 0;
+  bool has_est = false;
   DBuf<double> out;
   DBuf<unsigned long long> hits;
   DBuf<unsigned long long> counter;  // [0] work counter, [1..5] stats
@@ -170,6 +176,37 @@ namespace {
 
 constexpr int kGuideN = 4096;
 constexpr int kMaxBatchScenes = 4096;
+
+// compute_I_norm (detector.cpp:105-118): integral of (|S1|^2 + |S2|^2) sin(theta)
+// by the midpoint rule on 2048 points, S from the species' uniform-theta
+// tables as DataTable::Sample reads them (table.cpp:100-118).
+double i_norm_of(const lmc_species_desc& d, const double* tables) {
+  const int n = 2048;
+  const double dth = M_PI / n, inv_dx = 1.0 / d.s_dtheta;
+  const double* a = tables + d.s1_offset;
+  const double* b = tables + d.s2_offset;
+  double acc = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double th = (j + 0.5) * dth;
+    const double pos = th * inv_dx;
+    double s1r, s1i, s2r, s2i;
+    if (pos <= 0.0) {
+      s1r = a[0], s1i = a[1], s2r = b[0], s2i = b[1];
+    } else if (pos >= static_cast<double>(d.s_n - 1)) {
+      const int l = 2 * (d.s_n - 1);
+      s1r = a[l], s1i = a[l + 1], s2r = b[l], s2i = b[l + 1];
+    } else {
+      const int i = static_cast<int>(pos);
+      const double t = pos - static_cast<double>(i);
+      s1r = a[2 * i] + t * (a[2 * i + 2] - a[2 * i]);
+      s1i = a[2 * i + 1] + t * (a[2 * i + 3] - a[2 * i + 1]);
+      s2r = b[2 * i] + t * (b[2 * i + 2] - b[2 * i]);
+      s2i = b[2 * i + 1] + t * (b[2 * i + 3] - b[2 * i + 1]);
+    }
+    acc += (s1r * s1r + s1i * s1i + s2r * s2r + s2i * s2i) * std::sin(th);
+  }
+  return acc * dth;
+}
 
 template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if (T.ready) return LMC_OK;
@@ -219,6 +256,8 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     if (d.s_n < 2) return fail(LMC_ERR_INVALID, "species S table needs >= 2 nodes");
     o.s_n = d.s_n;
     o.inv_ds = static_cast<R>(1.0 / d.s_dtheta);
+    o.pi_inorm = static_cast<R>(M_PI * i_norm_of(d, s->tables.data()));
+    o.alb = static_cast<R>(d.mu_s / (d.mu_a + d.mu_s));
     {
       std::vector<double> key(s->tables.data() + d.s1_offset, s->tables.data() + d.s1_offset + 2 * d.s_n);
       key.insert(key.end(), s->tables.data() + d.s2_offset, s->tables.data() + d.s2_offset + 2 * d.s_n);
@@ -298,6 +337,11 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if ((rc = T.layers.upload(L.data(), L.size()))) return rc;
   if ((rc = T.species.upload(sp.data(), sp.size()))) return rc;
   if ((rc = T.lcs.upload(lc.data(), lc.size()))) return rc;
+  {
+    std::vector<R> eg(s->est_geom_host.begin(), s->est_geom_host.end());
+    if (eg.empty()) eg.push_back(R(0));
+    if ((rc = T.est_geom.upload(eg.data(), eg.size()))) return rc;
+  }
   if (cdf.empty()) cdf.push_back(R(1));
   if (guide.empty()) guide.assign(2, 0);
   if (sel.empty()) sel.push_back(R(1));
@@ -362,6 +406,12 @@ int validate(const lmc_scene_desc* d) {
     }
     if (need != s.out_len) return fail(LMC_ERR_INVALID, "sensor out_len does not match its grid");
     if (s.n[0] < 0 || s.n[1] < 0 || s.n[2] < 0) return fail(LMC_ERR_INVALID, "negative grid size");
+    if (s.n_phi_explicit != 0 &&
+        (s.kind != LMC_SENSOR_FARFIELD_CBS || s.n_phi_explicit != s.n[1] || s.phi_offset < 0 ||
+         s.phi_offset + s.n_phi_explicit > d->n_tables))
+      return fail(LMC_ERR_INVALID, "explicit phi slices out of bounds");
+    if (s.estimator && s.kind == LMC_SENSOR_FARFIELD_CBS && s.z != 0.0)
+      return fail(LMC_ERR_INVALID, "the CBS estimator is defined for a sensor at z = 0");
   }
   return LMC_OK;
 }
@@ -506,18 +556,41 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   // forward runs) when one 1024-thread workgroup can; otherwise 256-thread
   // workgroups read them through the caches.
   const size_t need = base_lds + (batch ? scene_lds : tab_lds);
+  const bool est = s->has_est;
+  if (est && (batch || kAdj))
+    return fail(LMC_ERR_UNSUPPORTED, "the CBS estimator runs single scenes without gradients");
+  if (est && !r->track_reverse_paths)
+    return fail(LMC_ERR_INVALID, "the CBS estimator needs track_reverse_paths (reverse path of every forced scatter)");
   const bool fit512 = 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
   const bool wide_ok = !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
   int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : (wide_ok ? 1024 : 256));
+  if (est) block = 256;
   if (block % 64 != 0 || block > 1024) return fail(LMC_ERR_INVALID, "block_size must be a multiple of 64, <= 1024");
   if (block > 512 && !wide_ok)
     return fail(LMC_ERR_UNSUPPORTED, "1024-thread workgroups need FP32 forward runs with LDS-resident tables");
-  const int bpc = r->blocks_per_cu > 0 ? r->blocks_per_cu : std::max(1, 1024 / block);
-  const int grid = g_num_cu * bpc;
-  const bool lds_tables =
+  int bpc = r->blocks_per_cu > 0 ? r->blocks_per_cu : std::max(1, 1024 / block);
+  bool lds_tables =
       block > 512 ||
       (need * static_cast<size_t>(std::min(bpc, std::max(1, 1024 / block))) <= g_max_lds_cu && need <= g_max_lds_block);
-  const size_t lds = lds_tables ? need : base_lds;
+  size_t lds = lds_tables ? need : base_lds;
+  void (*kt_est)(KParams<R>) = nullptr;
+  if (est) {
+    // estimator runs: three 256-thread workgroups per CU (the kernel's
+    // register budget), the estimator grids in LDS, the tables beside them
+    // when all three copies fit
+    const size_t est_need = base_lds + al16(sizeof(double) * s->est_lds_n);
+    lds_tables = 3 * (est_need + tab_lds) <= g_max_lds_cu && est_need + tab_lds <= g_max_lds_block;
+    lds = lds_tables ? est_need + tab_lds : est_need;
+    if constexpr (!kAdj) kt_est = lds_tables ? k_transport_est<R, kXo, true> : k_transport_est<R, kXo, false>;
+    int rcx = raise_lds_limit(kt_est, lds);
+    if (rcx) return rcx;
+    int per_cu = 0;
+    LMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kt_est, block, lds));
+    bpc = std::max(1, per_cu);
+    p.est_geom = T.est_geom.p;
+    p.est_lds_n = s->est_lds_n;
+  }
+  const int grid = g_num_cu * bpc;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   const bool track = r->track_reverse_paths != 0;
   int64_t chunk = static_cast<int64_t>(total);
@@ -538,7 +611,9 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   }
   p.q_count = s->qcount.p;
   void (*kt)(KParams<R>) = nullptr;
-  if (block > 512) {
+  if (est) {
+    kt = kt_est;
+  } else if (block > 512) {
     if constexpr (sizeof(R) == 4 && !kAdj) {
       if (batch)
         kt = track ? k_transport_wide<R, kXo, true, kAdj, true> : k_transport_wide<R, kXo, false, kAdj, true>;
@@ -564,6 +639,11 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   const size_t lds_r = ((sizeof(double) * lds_vals + 15) & ~15ull) + sizeof(unsigned) * (p.n_sensors + 1);
   if (lds_r > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "too many sensors for the sensor kernel");
   if ((rc = raise_lds_limit(kr, lds_r))) return rc;
+  // one resident round of sensor workgroups (a partial second round left the
+  // tail of the grid-stride loop on a quarter of the machine)
+  int kr_per_cu = 0;
+  LMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&kr_per_cu, kr, 256, lds_r));
+  const unsigned kr_grid = static_cast<unsigned>(g_num_cu * std::max(1, kr_per_cu));
   // the launch sequence: [zero work counter + record count, transport, sensors] per launch
   struct Launch { uint64_t first, n; };
   std::vector<Launch> seq;
@@ -616,7 +696,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
       for (const Launch& L : seq) {
         if ((crc = enqueue(L))) break;
         hipLaunchKernelGGL(kt, dim3(grid), dim3(block), lds, s->stream, p);
-        if (p.q_cap > 0) hipLaunchKernelGGL(kr, dim3(g_num_cu * 4), dim3(256), lds_r, s->stream, p);
+        if (p.q_cap > 0) hipLaunchKernelGGL(kr, dim3(kr_grid), dim3(256), lds_r, s->stream, p);
       }
       hipGraph_t g = nullptr;
       const hipError_t ce = hipStreamEndCapture(s->stream, &g);
@@ -651,7 +731,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
       LMC_HIP(hipGetLastError());
       LMC_HIP(hipEventRecord(s->ev1, s->stream));
       if (p.q_cap > 0) {
-        hipLaunchKernelGGL(kr, dim3(g_num_cu * 4), dim3(256), lds_r, s->stream, p);
+        hipLaunchKernelGGL(kr, dim3(kr_grid), dim3(256), lds_r, s->stream, p);
         LMC_HIP(hipGetLastError());
       }
       LMC_HIP(hipEventRecord(s->ev2, s->stream));
@@ -843,8 +923,41 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       o.n2 = sd.n[2];
       o.out_off = sc.out_base + sd.out_offset;
       queue_bounds(o);
+      o.est = sd.estimator ? 1 : 0;
+      o.est_lds = -1;
+      if (o.est) {
+        // no crossings reach an estimator-mode sensor (detector.cpp:246-253)
+        o.absorb = 0;
+        o.q_cmin = 2.0f;
+        if (sd.kind == LMC_SENSOR_FARFIELD_CBS) {
+          // bin centres and theta-band solid angles (detector.cpp:884-907),
+          // explicit phi columns (set_phi_slices, :909-943), theta cap (:1382)
+          const double theta_max = sd.g[0], dth = sd.g[3], dph = sd.g[4];
+          const double th_cap = sd.theta_pp_max > 0.0 ? std::min(sd.theta_pp_max, theta_max) : theta_max;
+          o.est_imax = std::min(sd.n[0], static_cast<int>(th_cap / dth));
+          o.est_geom = static_cast<int32_t>(s->est_geom_host.size());
+          auto& eg = s->est_geom_host;
+          for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::cos((it + 0.5) * dth));
+          for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::sin((it + 0.5) * dth));
+          for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::cos(it * dth) - std::cos((it + 1) * dth));
+          for (int jp = 0; jp < sd.n[1]; ++jp) {
+            const double ph = sd.n_phi_explicit > 0 ? d->tables[sd.phi_offset + jp] : (jp + 0.5) * dph;
+            eg.push_back(std::cos(ph));
+          }
+          for (int jp = 0; jp < sd.n[1]; ++jp) {
+            const double ph = sd.n_phi_explicit > 0 ? d->tables[sd.phi_offset + jp] : (jp + 0.5) * dph;
+            eg.push_back(std::sin(ph));
+          }
+          // the transport kernel's LDS copy of the grid when it is small (<= 48 KiB in all)
+          if ((s->est_lds_n + sd.out_len) * 8 <= 49152) {
+            o.est_lds = s->est_lds_n;
+            s->est_lds_n += static_cast<int>(sd.out_len);
+          }
+          s->has_est = true;
+        }
+      }
       // privatise small angular grids per workgroup of the sensor kernel (<= 48 KiB)
-      const bool small = privatise &&
+      const bool small = privatise && !o.est &&
                          (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_FARFIELD_FLUENCE ||
                           sd.kind == LMC_SENSOR_REFLECTANCE) &&
                          (s->lds_grid + sd.out_len) * 8 <= 49152;
@@ -856,8 +969,8 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       s->sensors_host.push_back(o);
       s->sensor_scene.push_back(scene_idx);
       P.count++;
-      if (sd.kind == LMC_SENSOR_FARFIELD_CBS) P.has_cbs = 1;
-      if (!sd.absorb) all_absorb = false;
+      if (sd.kind == LMC_SENSOR_FARFIELD_CBS && !o.est) P.has_cbs = 1;
+      if (!o.absorb) all_absorb = false;
     }
     // a plane on or outside the stack's faces is crossed only by a photon
     // leaving the sample; an interior plane whose sensors all absorb records

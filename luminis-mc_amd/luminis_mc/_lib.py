@@ -70,6 +70,7 @@ class SensorDesc(ctypes.Structure):
         ("x_min", c_double), ("x_max", c_double), ("y_min", c_double), ("y_max", c_double),
         ("g", c_double * 8), ("n", c_int32 * 4),
         ("out_offset", c_int64), ("out_len", c_int64),
+        ("estimator", c_int32), ("n_phi_explicit", c_int32), ("phi_offset", c_int64), ("theta_pp_max", c_double),
     ]
 
 
@@ -161,7 +162,7 @@ def lib():
                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     L.lmc_probe_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_uint64, c_int32,
                                    ctypes.POINTER(ctypes.c_uint32)]
-    if L.lmc_abi_version() != 1:
+    if L.lmc_abi_version() != 2:
         raise ImportError("libluminis_hip.so ABI version mismatch")
     _lib = L
     return L

@@ -126,6 +126,7 @@ class Sensor:
     def _fill_desc(self, d: _lib.SensorDesc) -> None:
         d.kind = self.kind
         d.absorb = int(self.absorb_photons)
+        d.estimator = int(self.estimator_enabled)
         d.z = self.origin.z
         d.f_dir_en = int(self.filter_direction_enabled)
         d.f_dir = int(self.filter_direction)
@@ -171,7 +172,18 @@ class FarFieldCBSSensor(Sensor):
         self.buffer = np.zeros(8 * self.N_t * self.N_theta * self.N_phi)
 
     def set_phi_slices(self, phi_angles) -> None:
-        raise NotImplementedError("explicit phi slices belong to the estimator mode, which this path does not run")
+        """Explicit phi columns, sorted and de-duplicated; resets the grids
+        (detector.cpp:909-943)."""
+        angles = [float(a) for a in phi_angles]
+        if not angles:
+            raise ValueError("set_phi_slices: phi_angles must be non-empty")
+        for a in angles:
+            if a < 0.0 or a > 2.0 * math.pi:
+                raise ValueError("set_phi_slices: every angle must be in [0, 2*pi]")
+        self.phi_values = sorted(set(angles))
+        self.phi_explicit = True
+        self.N_phi = len(self.phi_values)
+        self.buffer = np.zeros(8 * self.N_t * self.N_theta * self.N_phi)
 
     def grids(self) -> np.ndarray:
         return self.buffer.reshape(8, self.N_t, self.N_theta, self.N_phi)
@@ -190,6 +202,7 @@ class FarFieldCBSSensor(Sensor):
         for i, v in enumerate([self.theta_max, self.phi_max, self.t_max, self.dtheta, self.dphi, self.dt]):
             d.g[i] = v
         d.n[0], d.n[1], d.n[2] = self.N_theta, self.N_phi, self.N_t
+        d.theta_pp_max = self.theta_pp_max
 
 
 class PlanarFluenceSensor(Sensor):

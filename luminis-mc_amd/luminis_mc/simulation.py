@@ -62,9 +62,6 @@ class SceneBuild:
     def __init__(self, sample: Sample, laser: Laser, sensors: list[Sensor]):
         if not sample.layers:
             raise ValueError("Sample has no layers")
-        for s in sensors:
-            if s.estimator_enabled:
-                raise NotImplementedError("estimator-mode sensors are not run by the GPU transport path")
         tables: list[np.ndarray] = []
         n_tab = 0
 
@@ -147,6 +144,9 @@ class SceneBuild:
         for i, s in enumerate(sensors):
             d = se_arr[i]
             s._fill_desc(d)
+            if getattr(s, "phi_explicit", False):  # set_phi_slices columns
+                d.n_phi_explicit = len(s.phi_values)
+                d.phi_offset = put(np.asarray(s.phi_values, dtype=np.float64))
             d.out_offset = n_out
             d.out_len = s.out_len()
             self.offsets.append((n_out, s.out_len()))

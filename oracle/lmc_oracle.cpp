@@ -16,6 +16,8 @@
 //   src/detector.cpp:780-841     PlanarFluenceSensor::process_hit
 //   src/detector.cpp:1080-1310   reverse_field_prefix/suffix, time_bin,
 //                                accumulate_stokes, FarFieldCBSSensor::process_hit
+//   src/detector.cpp:36-161,1012-1044,1315-1566  the CBS estimator
+//                                (process_estimation and its helpers)
 //   src/sample.cpp:90-104,221-288 layer navigation, mixture species draw
 //   src/medium.cpp:123-126,175-191 free path, S(theta) lookup
 //   src/phase.cpp:27-49,108-115  azimuth rejection sampler, HG inversion
@@ -186,6 +188,7 @@ struct Species {
   std::vector<double> cdf;  // SamplingTable cdf (values are implicit)
   std::vector<cplx> s1, s2;
   double inv_ds{0};
+  double i_norm{0};  // compute_I_norm (detector.cpp:105-118)
 };
 
 // SamplingTable::Sample (src/table.cpp:50-66): lower_bound, then lerp.
@@ -245,6 +248,8 @@ struct Layer {
 
 struct Sensor {
   lmc_sensor_desc d;
+  // FarFieldCBS bin geometry (detector.cpp:884-907, 909-943)
+  std::vector<double> cos_th, sin_th, domega, cos_ph, sin_ph;
 };
 
 struct Scene {
@@ -356,12 +361,8 @@ bool check_conditions(const lmc_sensor_desc& d, V3 hit, V3 dir, int cross_dir, i
   return true;
 }
 
-// reverse_field_prefix / suffix (detector.cpp:1080-1161).
-CV reverse_field(const Scene& S, const Photon& ph) {
-  const M3& P0 = ph.P0;
-  const M3& Pnm1 = ph.Pn1;
-  const M3& P1 = ph.P1;
-  const M3& Pn = ph.Pn;
+// reverse_field_prefix (detector.cpp:1080-1121): stages A and B.
+CV reverse_prefix(const Scene& S, const M3& P0, const M3& Pnm1, const C2& Traw, int sp_n, const CV& E_in) {
   const V3 s0 = P0.row(2), snm1 = Pnm1.row(2);
   const V3 m0 = P0.row(0), n0 = P0.row(1);
   const V3 mnm1 = Pnm1.row(0), nnm1 = Pnm1.row(1);
@@ -377,7 +378,7 @@ CV reverse_field(const Scene& S, const Photon& ph) {
   C2 Sa;
   {
     cplx s2, s1;
-    smatrix(S.species[ph.last_sp], th_a, s2, s1);
+    smatrix(S.species[sp_n], th_a, s2, s1);
     Sa(0, 0) = s2;
     Sa(0, 1) = 0;
     Sa(1, 0) = 0;
@@ -387,17 +388,22 @@ CV reverse_field(const Scene& S, const Photon& ph) {
   const V3 mprime_in = safe_unit(cross_unit(nprime, s_in_a), m0);
   const V3 mprime_out = safe_unit(cross_unit(nprime, s_out_a), mnm1);
   C2 SRa = matcmul(Sa, rot2(mprime_in, nprime, m0, n0));
-  CV E = apply2(SRa, ph.E0);
+  CV E = apply2(SRa, E_in);
   E = apply2(rot2(mnm1, nnm1 * (-1.0), mprime_out, nprime), E);
   // Stage B: E <- Q T^T Q E
   C2 Tt;
-  Tt(0, 0) = ph.Traw(0, 0);
-  Tt(0, 1) = ph.Traw(1, 0);
-  Tt(1, 0) = ph.Traw(0, 1);
-  Tt(1, 1) = ph.Traw(1, 1);
+  Tt(0, 0) = Traw(0, 0);
+  Tt(0, 1) = Traw(1, 0);
+  Tt(1, 0) = Traw(0, 1);
+  Tt(1, 1) = Traw(1, 1);
   E = apply2(Q, E);
   E = apply2(Tt, E);
   E = apply2(Q, E);
+  return E;
+}
+
+// reverse_field_suffix (detector.cpp:1123-1161): stage C and normalisation.
+CV reverse_suffix(const Scene& S, const M3& P1, const M3& Pn, int sp_1, CV E) {
   // Stage C: scatter at r_1, -s1 -> s_n
   const V3 s1 = P1.row(2), sn = Pn.row(2);
   const V3 m1 = P1.row(0), n1 = P1.row(1);
@@ -408,7 +414,7 @@ CV reverse_field(const Scene& S, const Photon& ph) {
   C2 Sc;
   {
     cplx s2, s1v;
-    smatrix(S.species[ph.first_sp], th_c, s2, s1v);
+    smatrix(S.species[sp_1], th_c, s2, s1v);
     Sc(0, 0) = s2;
     Sc(0, 1) = 0;
     Sc(1, 0) = 0;
@@ -429,6 +435,11 @@ CV reverse_field(const Scene& S, const Photon& ph) {
     E.n = 0;
   }
   return E;
+}
+
+// reverse_field (detector.cpp:1165-1173) for a detected photon.
+CV reverse_field(const Scene& S, const Photon& ph) {
+  return reverse_suffix(S, ph.P1, ph.Pn, ph.first_sp, reverse_prefix(S, ph.P0, ph.Pn1, ph.Traw, ph.last_sp, ph.E0));
 }
 
 inline void project_to_lab(const CV& E, const M3& P, cplx& Ex, cplx& Ey) {  // :1048-1053
@@ -509,6 +520,164 @@ void cbs_process_hit(const Scene& S, const lmc_sensor_desc& d, Photon& ph, const
   project_to_lab(Er, P, Erx, Ery);
   accumulate_stokes(g, plane, bin0, 0, Efx, Efy, Erx, Ery);
   if (t_idx >= 1) accumulate_stokes(g, plane, t_idx * tplane + bin0, t_idx, Efx, Efy, Erx, Ery);
+  hits += 1;
+}
+
+// ---------------------------------------------------------- estimator ----
+// optical_depth_to_detector (detector.cpp:53-101): sum of mu_t,i L_i along the
+// straight ray from `start` over the distance L_total.
+double optical_depth_to_detector(const Scene& S, V3 start, V3 dir, double L_total) {
+  double tau = 0.0, traveled = 0.0;
+  V3 p = start;
+  size_t layer = layer_index_at(S, p.z);
+  if (layer >= S.layers.size()) return 0.0;
+  if (S.layers.size() == 1) return S.layers[layer].d.mu_t * L_total;
+  const double dz = dir.z;
+  while (traveled < L_total - 1e-15) {
+    const double remaining = L_total - traveled;
+    const double mu = S.layers[layer].d.mu_t;
+    double seg = remaining;
+    if (std::abs(dz) > 1e-15) {
+      double zi;
+      if (next_interface(S, p.z, p.z + dz * remaining, zi)) {
+        const double sd = (zi - p.z) / dz;
+        if (sd > 1e-15 && sd < remaining) seg = sd;
+      }
+    }
+    tau += mu * seg;
+    p.x += dir.x * seg;
+    p.y += dir.y * seg;
+    p.z += dir.z * seg;
+    traveled += seg;
+    if (seg >= remaining - 1e-15) break;
+    const double nudge = (dz > 0) ? 1e-12 : -1e-12;
+    const size_t next = layer_index_at(S, p.z + nudge);
+    if (next >= S.layers.size()) break;
+    layer = next;
+  }
+  return tau;
+}
+
+// phase_F (detector.cpp:1012-1023).
+double phase_F(cplx s2, cplx s1, double c, double sn, const CV& E) {
+  const double s22 = std::norm(s2), s11 = std::norm(s1);
+  const double Emm = std::norm(E.m), Enn = std::norm(E.n);
+  const double c2 = c * c, s2q = sn * sn;
+  return Emm * (s22 * c2 + s11 * s2q) + Enn * (s22 * s2q + s11 * c2) +
+         2.0 * std::real(E.m * std::conj(E.n)) * (s22 - s11) * sn * c;
+}
+
+// apply_scatter_normalized (detector.cpp:121-161).
+CV apply_scatter_normalized(cplx s2, cplx s1, double c, double sn, const CV& Ein) {
+  C2 Sm;
+  Sm(0, 0) = s2;
+  Sm(0, 1) = 0;
+  Sm(1, 0) = 0;
+  Sm(1, 1) = s1;
+  C2 R;
+  R(0, 0) = c;
+  R(0, 1) = sn;
+  R(1, 0) = -sn;
+  R(1, 1) = c;
+  const C2 T = matcmul(Sm, R);
+  const double F = phase_F(s2, s1, c, sn, Ein);
+  if (F < 1e-300) return CV{0.0, 0.0};
+  const double inv = 1.0 / std::sqrt(F);
+  return CV{inv * (T(0, 0) * Ein.m + T(0, 1) * Ein.n), inv * (T(1, 0) * Ein.m + T(1, 1) * Ein.n)};
+}
+
+// scatter_frame (detector.cpp:1027-1044).
+M3 scatter_frame(const M3& P, double theta, double c, double sn) {
+  const double ct = std::cos(theta), st = std::sin(theta);
+  M3 A;
+  A(0, 0) = ct * c;
+  A(0, 1) = ct * sn;
+  A(0, 2) = -st;
+  A(1, 0) = -sn;
+  A(1, 1) = c;
+  A(1, 2) = 0;
+  A(2, 0) = st * c;
+  A(2, 1) = st * sn;
+  A(2, 2) = ct;
+  return matmul(A, P);
+}
+
+// FarFieldCBSSensor::process_estimation (detector.cpp:1315-1566): at the
+// vertex `ph` (species selected, not yet scattered) force the next scatter
+// into every (theta, phi) bin around the backscatter axis and deposit the
+// forward field and, from the second order on, the reverse-path partner.
+void cbs_process_estimation(const Scene& S, const Sensor& sen, const Photon& ph, double* g, uint64_t& hits) {
+  const lmc_sensor_desc& d = sen.d;
+  if (d.f_ev_en && (static_cast<int>(ph.events) < d.f_ev_min || static_cast<int>(ph.events) > d.f_ev_max)) return;
+  const Species& med = S.species[ph.active];
+  const Layer& lay = S.layers[ph.layer];
+  if (med.i_norm < 1e-300) return;
+  const V3 e1 = ph.P0.row(0), e2 = ph.P0.row(1), s_in = ph.P0.row(2);
+  const V3 e3 = s_in * (-1.0);
+  const V3 m_cur = ph.P.row(0), n_cur = ph.P.row(1), s_cur = ph.P.row(2);
+  const bool single = ph.events == 0;
+  const double w_scatter = single ? ph.weight * (lay.d.mu_s / lay.d.mu_t)
+                                  : ph.weight * (med.d.mu_s / (med.d.mu_a + med.d.mu_s));
+  if (w_scatter < 1e-300) return;
+  const double theta_max = d.g[0], dtheta = d.g[3];
+  const int Nth = d.n[0], Nph = d.n[1];
+  const double th_cap = (d.theta_pp_max > 0.0) ? std::min(d.theta_pp_max, theta_max) : theta_max;
+  const int i_max = std::min(Nth, static_cast<int>(th_cap / dtheta));
+  const double dphi = d.g[4];
+  const int64_t tplane = static_cast<int64_t>(Nth) * Nph;
+  const int64_t plane = tplane * d.n[2];
+  CV pre{0.0, 0.0};
+  if (!single) {
+    C2 Tmid = ph.Traw;
+    if (ph.events >= 2 && ph.has_T_prev) Tmid = matcmul(ph.Traw_buf, ph.Traw);
+    pre = reverse_prefix(S, ph.P0, ph.P, Tmid, ph.active, ph.E0);
+  }
+  for (int it = 0; it < i_max; ++it) {
+    const double dOmega_band = sen.domega[it], s_th = sen.sin_th[it], c_th = sen.cos_th[it];
+    for (int jp = 0; jp < Nph; ++jp) {
+      const double c_ph = sen.cos_ph[jp], s_ph = sen.sin_ph[jp];
+      const V3 s_out = e1 * (s_th * c_ph) + e2 * (s_th * s_ph) + e3 * c_th;
+      // intersect_plane (detector.cpp:36-47) with the plane z = d.z, normal +z
+      const double denom = s_out.z;
+      if (std::abs(denom) < 1e-14) continue;
+      const double L = (d.z - ph.pos.z) / denom;
+      if (L <= 0.0) continue;
+      const double Tr = std::exp(-optical_depth_to_detector(S, ph.pos, s_out, L));
+      if (Tr < 1e-20) continue;
+      const V3 pv = cross_unit(s_cur, s_out);
+      const double pn = norm(pv);
+      if (pn < 1e-12) continue;
+      const V3 p_hat = pv * (1.0 / pn);
+      const double sin_phi = -dot(m_cur, p_hat);
+      const double cos_phi = dot(n_cur, p_hat);
+      const double th_scat = std::acos(clamp_pm1(dot(s_cur, s_out)));
+      cplx s2, s1;
+      smatrix(med, th_scat, s2, s1);
+      const double F = phase_F(s2, s1, cos_phi, sin_phi, ph.E);
+      if (F < 1e-300) continue;
+      const double prob_bin = (F / (kPi * med.i_norm)) * (dOmega_band * dphi);
+      const double w_bin = w_scatter * Tr * prob_bin;
+      if (w_bin < 1e-300) continue;
+      const cplx amp = std::sqrt(w_bin) * std::exp(cplx(0.0, ph.k * L));
+      const M3 P_exit = scatter_frame(ph.P, th_scat, cos_phi, sin_phi);
+      const CV Ef_loc = apply_scatter_normalized(s2, s1, cos_phi, sin_phi, ph.E);
+      const int t_idx = cbs_time_bin(d, ph.launch_time + (ph.opticalpath + L) / ph.velocity);
+      if (t_idx < 0) continue;
+      const CV Ef{Ef_loc.m * amp, Ef_loc.n * amp};
+      cplx Efx, Efy, Erx(0.0, 0.0), Ery(0.0, 0.0);
+      project_to_lab(Ef, P_exit, Efx, Efy);
+      if (!single) {
+        const CV Er_loc = reverse_suffix(S, ph.P1, P_exit, ph.first_sp, pre);
+        const V3 qb = (s_out + s_in) * ph.k;
+        const cplx path_phase = std::exp(cplx(0.0, dot(qb, ph.pos - ph.r1)));
+        const CV Er{Er_loc.m * amp * path_phase, Er_loc.n * amp * path_phase};
+        project_to_lab(Er, P_exit, Erx, Ery);
+      }
+      const int64_t bin0 = static_cast<int64_t>(it) * Nph + jp;
+      accumulate_stokes(g, plane, bin0, 0, Efx, Efy, Erx, Ery);
+      if (t_idx >= 1) accumulate_stokes(g, plane, t_idx * tplane + bin0, t_idx, Efx, Efy, Erx, Ery);
+    }
+  }
   hits += 1;
 }
 
@@ -623,6 +792,7 @@ bool record_hit(const Scene& S, Photon& ph, Accum& acc) {
     info.opticalpath_at_hit = gpl_hit;
     for (int si : S.plane_sensors[p]) {
       const lmc_sensor_desc& sd = S.sensors[si].d;
+      if (sd.estimator) continue;  // detected only through the estimator (detector.cpp:246-253)
       const V3 dir = ph.P.row(2);
       if (!check_conditions(sd, hp, dir, cdir, static_cast<int>(ph.events))) continue;
       double* g = acc.out.data() + sd.out_offset;
@@ -780,6 +950,12 @@ void run_photon(const Scene& S, Photon& ph, Xoshiro& rng, Accum& acc, bool track
       sp += static_cast<int>(idx);
     }
     ph.active = sp;
+    // SensorsGroup::run_estimators (detector.cpp:272-281)
+    for (size_t si = 0; si < S.sensors.size(); ++si) {
+      const Sensor& sen = S.sensors[si];
+      if (sen.d.estimator && sen.d.kind == LMC_SENSOR_FARFIELD_CBS)
+        cbs_process_estimation(S, sen, ph, acc.out.data() + sen.d.out_offset, acc.hits[si]);
+    }
     const Species& med = S.species[sp];
     const double theta = sample_theta(med, rng.uniform());
     cplx s2, s1;
@@ -911,6 +1087,18 @@ Scene build_scene(const lmc_scene_desc* d) {
       sp.s2[j] = cplx(d->tables[sp.d.s2_offset + 2 * j], d->tables[sp.d.s2_offset + 2 * j + 1]);
     }
     sp.inv_ds = 1.0 / sp.d.s_dtheta;
+    {  // compute_I_norm (detector.cpp:105-118): midpoint rule, 2048 points
+      const int n = 2048;
+      const double dth = kPi / n;
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double th = (j + 0.5) * dth;
+        cplx s2, s1;
+        smatrix(sp, th, s2, s1);
+        acc += (std::norm(s1) + std::norm(s2)) * std::sin(th);
+      }
+      sp.i_norm = acc * dth;
+    }
     S.species.push_back(std::move(sp));
   }
   for (int i = 0; i < d->n_layers; ++i) {
@@ -925,6 +1113,20 @@ Scene build_scene(const lmc_scene_desc* d) {
   for (int i = 0; i < d->n_sensors; ++i) {
     Sensor s;
     s.d = d->sensors[i];
+    if (s.d.kind == LMC_SENSOR_FARFIELD_CBS) {  // detector.cpp:884-907, 909-943
+      const double dth = s.d.g[3], dph = s.d.g[4];
+      for (int it = 0; it < s.d.n[0]; ++it) {
+        const double th = (it + 0.5) * dth;
+        s.cos_th.push_back(std::cos(th));
+        s.sin_th.push_back(std::sin(th));
+        s.domega.push_back(std::cos(it * dth) - std::cos((it + 1) * dth));
+      }
+      for (int jp = 0; jp < s.d.n[1]; ++jp) {
+        const double ph = s.d.n_phi_explicit > 0 ? d->tables[s.d.phi_offset + jp] : (jp + 0.5) * dph;
+        s.cos_ph.push_back(std::cos(ph));
+        s.sin_ph.push_back(std::sin(ph));
+      }
+    }
     S.sensors.push_back(s);
     // z_layers map keyed with the 1e-9 comparator (detector.cpp:166-176)
     bool placed = false;
@@ -1197,6 +1399,16 @@ void oracle_reverse_field(const double* P0, const double* P1, const double* Pnm1
   out[1] = E.m.imag();
   out[2] = E.n.real();
   out[3] = E.n.imag();
+}
+
+// compute_I_norm of species `sp` of a scene (detector.cpp:105-118).
+double oracle_species_i_norm(const lmc_scene_desc* desc, int sp) {
+  try {
+    const Scene S = build_scene(desc);
+    return S.species.at(sp).i_norm;
+  } catch (...) {
+    return -1.0;
+  }
 }
 
 void oracle_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {

@@ -43,6 +43,8 @@ def lib():
         L.oracle_xoshiro_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _dp]
         L.oracle_xoshiro_u64.argtypes = [ctypes.c_uint64, ctypes.c_int, _u64p]
         L.oracle_mix_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_species_i_norm.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_species_i_norm.restype = ctypes.c_double
         L.oracle_mix_seed.restype = ctypes.c_uint64
         L.oracle_reverse_field.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                            _dp, _dp, _dp]
@@ -124,6 +126,11 @@ def reverse_field(P0, P1, Pnm1, Pn, T, E0, s_last, s_first, dtheta: float):
                                int(len(s_last[0])), float(dtheta), s1l.ctypes.data_as(_dp), s2l.ctypes.data_as(_dp),
                                s1f.ctypes.data_as(_dp), s2f.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
     return np.array([out[0] + 1j * out[1], out[2] + 1j * out[3]])
+
+
+def species_i_norm(scene_build, sp: int) -> float:
+    """compute_I_norm (detector.cpp:105-118) of species `sp` of a scene build."""
+    return float(lib().oracle_species_i_norm(ctypes.addressof(scene_build.desc), int(sp)))
 
 
 def philox(ctr, k0: int, k1: int):

@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
     L = ctypes.CDLL(_lib.LIB_PATH)
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
-    assert L.lmc_abi_version() == 1
+    assert L.lmc_abi_version() == 2
 
 
 def test_ctypes_structs_match_c_layout(tmp_path):
