@@ -75,7 +75,7 @@ def cpu_baseline(build, seconds: float):
                       f"(xoshiro per-photon streams, std::thread clone/merge), {dt:.1f} s wall"}
 
 
-def cpu_rate_scenes(builds, budgets, seconds: float, seed: int = 777):
+def cpu_rate_scenes(builds, budgets, seconds: float, seed: int = 777, calib_photons: int = 20000):
     """Oracle photons/s over several scenes, each run with photons in
     proportion to its budget (one std::thread pool per scene, all host cores),
     sized to about `seconds` of CPU work by a short calibration pass."""
@@ -94,7 +94,7 @@ def cpu_rate_scenes(builds, budgets, seconds: float, seed: int = 777):
             n_tot += n
         return n_tot, time.perf_counter() - t
 
-    n0, t0 = run(20000 * len(builds))
+    n0, t0 = run(calib_photons * len(builds))
     target = n0 / t0 * seconds
     n, dt = run(target)
     return {"value": round(n / dt, 1), "unit": "photons/s", "cores": threads, "kind": "port",
@@ -240,6 +240,36 @@ def polydisperse_leg(n: int = 200_000_000, cpu_seconds: float = 0.0):
     return res
 
 
+def estimator_leg(n: int = 1_000_000, cpu_seconds: float = 0.0):
+    """The CBS estimator (every vertex forced into each of 300 theta bins) on
+    the reference's estimator-study medium; the vertex-bin rate is the work unit
+    of the per-bin loop."""
+    import scenes
+    from luminis_mc.simulation import GpuScene, SimConfig, run_desc_from
+    sample, laser, sensors, keep = scenes.scene_estimator()
+    sc = GpuScene(sample, laser, sensors)
+    c = SimConfig()
+    c.seed, c.track_reverse_paths = 8080, True
+    sc.run(run_desc_from(c, n_photons=min(n, 20000)))
+    t = time.perf_counter()
+    st = sc.run(run_desc_from(c, n_photons=n, photon_offset=10 ** 7))
+    dt = time.perf_counter() - t
+    build = sc.build
+    sc.close()
+    bins = 300
+    res = {"workload": "estimator-mode FarFieldCBSSensor, 10 deg in 300 x 1 bins, semi-infinite RGD r=0.110 um, "
+                       "mu_s 10, mu_a 0 /mm, Gaussian beam (the reference's estimator study)",
+           "photons": n, "photons_per_s": round(n / dt, 1),
+           "events_per_photon": round(st["events"] / max(1, st["photons"]), 2),
+           "vertex_bins_per_s": round(st["events"] * bins / dt, 1), "kernel_ms": round(st["kernel_ms"], 3),
+           "launch": {"block": st["block_size"], "blocks_per_cu": st["blocks_per_cu"],
+                      "lds_tables": bool(st["lds_tables"])}}
+    if cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_rate_scenes([build], [1], cpu_seconds, calib_photons=64)
+        res["speedup_vs_cpu"] = round(res["photons_per_s"] / res["cpu_baseline"]["value"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +282,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-poly", action="store_true")
+    ap.add_argument("--no-estimator", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many MiB device-to-device (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -324,6 +355,8 @@ def main():
         out["adjoint"] = adjoint_leg()
     if not args.no_poly:
         out["polydisperse"] = polydisperse_leg(cpu_seconds=0.0 if args.no_cpu else 8.0)
+    if not args.no_estimator:
+        out["estimator"] = estimator_leg(cpu_seconds=0.0 if args.no_cpu else 10.0)
     if not args.no_cpu:
         cb = cpu_baseline(scene.build, args.cpu_seconds)
         out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}

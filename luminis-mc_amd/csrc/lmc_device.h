@@ -272,7 +272,7 @@ template <class R> struct SpeciesDev {
   int32_t guide_n;
   int32_t s_n;
   int32_t s_off;     // offset in the S pool (one STab entry per node)
-  int32_t pad;
+  int32_t guide_shift;  // guide entries hold lower_bound >> guide_shift (tables beyond 65536 nodes)
   R cdf_min, cdf_step, hg_g, inv_ds;
   R pi_inorm;  // pi * I_norm, the estimator's angular normalisation (detector.cpp:105-118, 1436)
   R alb;       // mu_s / (mu_a + mu_s) of the species (estimator weight, detector.cpp:1466)

@@ -134,6 +134,7 @@ template <class R, bool kXo> struct Photon {
   R w, gpl, t0;
   double opl;
   int32_t events, layer;
+  int32_t cur_sp;    // species of the pending vertex between the two parts of an estimator event
   Xoshiro128ss rng;  // this photon's event stream (seeded from Philox at launch)
   bool alive;
   // CBS history
@@ -236,15 +237,19 @@ __device__ __forceinline__ void azimuth_sample(R a, R b, R c, uint32_t wv, uint3
 }
 
 // Inverse-CDF lookup: SamplingTable::Sample (src/table.cpp:50-66).  The
-// lower_bound is narrowed by a guide table g[j] = lower_bound(j/G), so the
-// result is exactly the reference's index on the same CDF values.
+// lower_bound is narrowed by a guide table g[j] = lower_bound(j/G) (stored
+// shifted right by guide_shift for tables beyond 65536 nodes, the bracket
+// widened to match), so the result is exactly the reference's index on the
+// same CDF values.
 template <class R>
 __device__ __forceinline__ R table_sample(const SceneView<R>& v, const SpeciesDev<R>& s, R u) {
   const R* c = v.cdf + s.cdf_off;
   const uint16_t* g = v.guide + s.guide_off;
   int j = static_cast<int>(u * static_cast<R>(s.guide_n));
   j = j < 0 ? 0 : (j >= s.guide_n ? s.guide_n - 1 : j);
-  int lo = g[j], hi = g[j + 1];
+  const int sh = s.guide_shift;
+  int lo = static_cast<int>(g[j]) << sh, hi = (static_cast<int>(g[j + 1]) << sh) + ((1 << sh) - 1);
+  hi = hi < s.cdf_n - 1 ? hi : s.cdf_n - 1;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (c[mid] < u)
@@ -676,31 +681,48 @@ __device__ __forceinline__ R est_optical_depth(const SceneView<R>& v, int n_laye
   return tau;
 }
 
-// FarFieldCBSSensor::process_estimation (detector.cpp:1315-1566) at one
-// vertex: the photon's next scatter is forced into every (theta, phi) bin of
-// the sensor, with the weight w * albedo * exp(-tau) * F / (pi I_norm) * dOmega
-// that lands there, the forward field of that scatter and, from the second
-// order on, its reverse-path partner; the reverse prefix (stages A, B) is
-// computed once per vertex.  The common phase exp(i k L) of the forward and
-// reverse fields cancels in every Stokes term and is not formed.  The lanes of
-// a wave start at different bins, so their LDS adds rarely meet.
+// Lane l's value in every lane (a wave-uniform value the compiler keeps in
+// scalar registers).
+__device__ __forceinline__ float bcast(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double bcast(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ int bcast(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+template <class R> __device__ __forceinline__ V3<R> bcast(V3<R> a, int l) {
+  return {bcast(a.x, l), bcast(a.y, l), bcast(a.z, l)};
+}
+template <class R> __device__ __forceinline__ cx<R> bcast(cx<R> a, int l) { return {bcast(a.r, l), bcast(a.i, l)}; }
+
+// FarFieldCBSSensor::process_estimation (detector.cpp:1315-1566) for the
+// vertices of a wave: every lane whose photon stands at a scattering vertex
+// (species chosen, not yet scattered) gets its next scatter forced into every
+// (theta, phi) bin of the sensor, with the weight w * albedo * exp(-tau) *
+// F / (pi I_norm) * dOmega that lands there, the forward field of that
+// scatter and, from the second order on, its reverse-path partner.  Each lane
+// computes its vertex's reverse prefix (stages A, B) once; then the wave takes
+// the vertices one at a time, broadcast to scalar registers, with the bins
+// spread over the lanes (distinct bins per lane: no LDS add conflicts within a
+// wave, and a long photon no longer runs its bins alone).  The common phase
+// exp(i k L) of the forward and reverse fields cancels in every Stokes term
+// and is not formed.
 template <class R, bool kXo>
-__device__ void estimate_vertex(const KParams<R>& p, const SceneView<R>& v, const Photon<R, kXo>& ph, int si,
-                                const SpeciesDev<R>& S, R alb_layer) {
+__device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const Photon<R, kXo>& ph, int si) {
   const SensorDev& sd = v.sens[si];
-  if (sd.f_ev_en && (ph.events < sd.f_ev_min || ph.events > sd.f_ev_max)) return;
   const R tiny = static_cast<R>(sizeof(R) == 8 ? 1e-300 : 0.0);
-  if (!(S.pi_inorm > R(0))) return;
+  bool mine = ph.alive;
+  if (mine && sd.f_ev_en && (ph.events < sd.f_ev_min || ph.events > sd.f_ev_max)) mine = false;
+  const SpeciesDev<R>& S = v.S[mine ? ph.cur_sp : 0];
   const bool single = ph.events == 0;
-  const R w_sc = ph.w * (single ? alb_layer : S.alb);
-  if (w_sc < tiny) return;
+  const R w_sc = ph.w * (single ? v.L[mine ? ph.layer : 0].albedo : S.alb);
+  if (mine && (!(S.pi_inorm > R(0)) || w_sc < tiny)) mine = false;
   const LaunchConst<R>& lc = *v.lc;
-  const V3<R> e1{lc.lm[0], lc.lm[1], lc.lm[2]}, e2{lc.ln[0], lc.ln[1], lc.ln[2]};
-  const V3<R> s_in{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
-  const Frame<R>& Pc = ph.P;
   cx<R> pm{R(0), R(0)}, pn{R(0), R(0)};
-  Frame<R> P1;
-  if (!single) {
+  if (mine && !single) {
     cx<R> Tm[4] = {ph.T[0], ph.T[1], ph.T[2], ph.T[3]};
     if (ph.events >= 2) {  // interior product of the estimated path: J_k ... J_2
       const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
@@ -712,10 +734,15 @@ __device__ void estimate_vertex(const KParams<R>& p, const SceneView<R>& v, cons
       Tm[2] = cmul(ph.b1, r2);
       Tm[3] = cmul(ph.b1, r3);
     }
-    reverse_prefix(lc, v.stab, S, Pc, Tm, pm, pn);
-    P1 = frame_of(ph.m1, ph.s1);
+    reverse_prefix(lc, v.stab, S, ph.P, Tm, pm, pn);
   }
-  const SpeciesDev<R>& S1 = v.S[ph.sp >> 16];
+  uint64_t todo = __ballot(mine);
+  if (todo == 0) return;
+  const int lane = __lane_id();
+  if (lane == __ffsll(static_cast<unsigned long long>(todo)) - 1)
+    atomicAdd(&v.lhits[si], static_cast<unsigned>(__popcll(todo)));  // hits += 1 per estimated vertex
+  const V3<R> e1{lc.lm[0], lc.lm[1], lc.lm[2]}, e2{lc.ln[0], lc.ln[1], lc.ln[2]};
+  const V3<R> s_in{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
   const int Nth = sd.n0, Nph = sd.n1;
   const R* gc = p.est_geom + sd.est_geom;  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each)
   const R dphi = static_cast<R>(sd.g[4]);
@@ -726,91 +753,106 @@ __device__ void estimate_vertex(const KParams<R>& p, const SceneView<R>& v, cons
   double* lg = in_lds ? v.lgrid + sd.est_lds : nullptr;
   double* hg = p.out + sd.out_off;
   const int nb = sd.est_imax * Nph;
-  if (nb <= 0) return;
-  int b = static_cast<int>(__lane_id()) % nb;
-  for (int k = 0; k < nb; ++k, b = (b + 1 == nb) ? 0 : b + 1) {
-    const int it = b / Nph, jp = b - it * Nph;
-    const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
-    const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
-    const R a1 = s_th * c_ph, a2 = s_th * s_ph;
-    const V3<R> so{e1.x * a1 + e2.x * a2 - s_in.x * c_th, e1.y * a1 + e2.y * a2 - s_in.y * c_th,
-                   e1.z * a1 + e2.z * a2 - s_in.z * c_th};
-    // intersect_plane (detector.cpp:36-47) with the sensor plane, normal +z
-    if (M<R>::fabs(so.z) < static_cast<R>(1e-14)) continue;
-    const R L = (zp - ph.pos.z) / so.z;
-    if (!(L > R(0))) continue;
-    const R Tr = M<R>::exp(-est_optical_depth(v, p.n_layers, ph.layer, ph.pos, so, L));
-    if (Tr < static_cast<R>(1e-20)) continue;
-    const V3<R> pv = cross_unit(Pc.s, so);
-    const R pnrm = M<R>::sqrt(dot(pv, pv));
-    if (pnrm < static_cast<R>(1e-12)) continue;
-    const V3<R> ph_hat = vscale(pv, R(1) / pnrm);
-    const R sin_phi = -dot(Pc.m, ph_hat), cos_phi = dot(Pc.n, ph_hat);
-    const R th = M<R>::acos(clamp1(dot(Pc.s, so)));
-    cx<R> s1, s2;
-    s_lookup(v.stab, S, th, s1, s2);
-    // phase_F (detector.cpp:1012-1023)
-    const R s22 = cnorm(s2), s11 = cnorm(s1), Emm = cnorm(ph.Em), Enn = cnorm(ph.En);
-    const R cc = cos_phi * cos_phi, ss = sin_phi * sin_phi;
-    const R F = Emm * (s22 * cc + s11 * ss) + Enn * (s22 * ss + s11 * cc) +
-                R(2) * (ph.Em.r * ph.En.r + ph.Em.i * ph.En.i) * (s22 - s11) * sin_phi * cos_phi;
-    if (F < tiny) continue;
-    const R w_bin = w_sc * Tr * ((F / S.pi_inorm) * (dom * dphi));
-    if (w_bin < tiny) continue;
-    int t_idx = 0;  // time_bin (detector.cpp:1182-1190); photon speed 1 in path units
-    if (dt > R(0)) {
-      const R arr = ph.t0 + (ph.gpl + L);
-      if (arr < R(0) || arr >= tmax) continue;
-      t_idx = static_cast<int>(arr / dt) + 1;
-      if (t_idx >= sd.n2) continue;
-    }
-    const R amp = M<R>::sqrt(w_bin);
-    // exit frame (scatter_frame, detector.cpp:1027-1044) and the normalised
-    // forward field (apply_scatter_normalized, :121-161)
-    R st, ct;
-    M<R>::sincos(th, &st, &ct);
-    Frame<R> Pe;
-    {
-      const R ctc = ct * cos_phi, cts = ct * sin_phi, stc = st * cos_phi, sts = st * sin_phi;
-      Pe.m = {ctc * Pc.m.x + cts * Pc.n.x - st * Pc.s.x, ctc * Pc.m.y + cts * Pc.n.y - st * Pc.s.y,
-              ctc * Pc.m.z + cts * Pc.n.z - st * Pc.s.z};
-      Pe.n = {-sin_phi * Pc.m.x + cos_phi * Pc.n.x, -sin_phi * Pc.m.y + cos_phi * Pc.n.y,
-              -sin_phi * Pc.m.z + cos_phi * Pc.n.z};
-      Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
-              stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
-    }
-    const R fi = amp / M<R>::sqrt(F);
-    const cx<R> um = cadd(cscale(ph.Em, cos_phi), cscale(ph.En, sin_phi));
-    const cx<R> un = cadd(cscale(ph.Em, -sin_phi), cscale(ph.En, cos_phi));
-    const cx<R> fm = cscale(cmul(s2, um), fi), fn = cscale(cmul(s1, un), fi);
-    const cx<R> Efx = cadd(cscale(fm, Pe.m.x), cscale(fn, Pe.n.x));
-    const cx<R> Efy = cadd(cscale(fm, Pe.m.y), cscale(fn, Pe.n.y));
-    cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
-    if (!single) {
-      cx<R> rm, rn;
-      reverse_suffix(v.stab, S1, P1, Pe, pm, pn, rm, rn);
-      // CBS path phase exp(i k (s_out + s_in) . (r_vertex - r_1))
-      const R arg = lc.k0 * ((so.x + s_in.x) * (ph.pos.x - ph.r1.x) + (so.y + s_in.y) * (ph.pos.y - ph.r1.y) +
-                             (so.z + s_in.z) * (ph.pos.z - ph.r1.z));
-      R sa, ca;
-      M<R>::sincos(arg, &sa, &ca);
-      const cx<R> pp{ca * amp, sa * amp};
-      rm = cmul(rm, pp);
-      rn = cmul(rn, pp);
-      Erx = cadd(cscale(rm, Pe.m.x), cscale(rn, Pe.n.x));
-      Ery = cadd(cscale(rm, Pe.m.y), cscale(rn, Pe.n.y));
-    }
-    const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
-    const int64_t off = t_idx * tplane + b;
+  while (todo) {
+    const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    todo &= todo - 1;
+    // lane l's vertex, wave-uniform
+    const V3<R> pos = bcast(ph.pos, l), r1 = bcast(ph.r1, l);
+    Frame<R> Pc;
+    Pc.m = bcast(ph.P.m, l);
+    Pc.n = bcast(ph.P.n, l);
+    Pc.s = bcast(ph.P.s, l);
+    const cx<R> Em = bcast(ph.Em, l), En = bcast(ph.En, l);
+    const cx<R> xm = bcast(pm, l), xn = bcast(pn, l);
+    const R w = bcast(w_sc, l), gpl = bcast(ph.gpl, l), t0 = bcast(ph.t0, l);
+    const int vsp = bcast(ph.cur_sp, l), vlayer = bcast(ph.layer, l), vev = bcast(ph.events, l);
+    const int vsp1 = bcast(ph.sp, l) >> 16;
+    const bool vsingle = vev == 0;
+    Frame<R> P1;
+    if (!vsingle) P1 = frame_of(bcast(ph.m1, l), bcast(ph.s1, l));
+    const SpeciesDev<R>& Sv = v.S[vsp];
+    const SpeciesDev<R>& S1 = v.S[vsp1];
+    const R Emm = cnorm(Em), Enn = cnorm(En), E12 = R(2) * (Em.r * En.r + Em.i * En.i);
+    for (int b = lane; b < nb; b += 64) {
+      const int it = b / Nph, jp = b - it * Nph;
+      const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
+      const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
+      const R a1 = s_th * c_ph, a2 = s_th * s_ph;
+      const V3<R> so{e1.x * a1 + e2.x * a2 - s_in.x * c_th, e1.y * a1 + e2.y * a2 - s_in.y * c_th,
+                     e1.z * a1 + e2.z * a2 - s_in.z * c_th};
+      // intersect_plane (detector.cpp:36-47) with the sensor plane, normal +z
+      if (M<R>::fabs(so.z) < static_cast<R>(1e-14)) continue;
+      const R L = (zp - pos.z) / so.z;
+      if (!(L > R(0))) continue;
+      const R Tr = M<R>::exp(-est_optical_depth(v, p.n_layers, vlayer, pos, so, L));
+      if (Tr < static_cast<R>(1e-20)) continue;
+      const V3<R> pv = cross_unit(Pc.s, so);
+      const R pnrm = M<R>::sqrt(dot(pv, pv));
+      if (pnrm < static_cast<R>(1e-12)) continue;
+      const V3<R> ph_hat = vscale(pv, R(1) / pnrm);
+      const R sin_phi = -dot(Pc.m, ph_hat), cos_phi = dot(Pc.n, ph_hat);
+      const R th = M<R>::acos(clamp1(dot(Pc.s, so)));
+      cx<R> s1, s2;
+      s_lookup(v.stab, Sv, th, s1, s2);
+      // phase_F (detector.cpp:1012-1023)
+      const R s22 = cnorm(s2), s11 = cnorm(s1);
+      const R cc = cos_phi * cos_phi, ss = sin_phi * sin_phi;
+      const R F = Emm * (s22 * cc + s11 * ss) + Enn * (s22 * ss + s11 * cc) + E12 * (s22 - s11) * sin_phi * cos_phi;
+      if (F < tiny) continue;
+      const R w_bin = w * Tr * ((F / Sv.pi_inorm) * (dom * dphi));
+      if (w_bin < tiny) continue;
+      int t_idx = 0;  // time_bin (detector.cpp:1182-1190); photon speed 1 in path units
+      if (dt > R(0)) {
+        const R arr = t0 + (gpl + L);
+        if (arr < R(0) || arr >= tmax) continue;
+        t_idx = static_cast<int>(arr / dt) + 1;
+        if (t_idx >= sd.n2) continue;
+      }
+      const R amp = M<R>::sqrt(w_bin);
+      // exit frame (scatter_frame, detector.cpp:1027-1044) and the normalised
+      // forward field (apply_scatter_normalized, :121-161)
+      R st, ct;
+      M<R>::sincos(th, &st, &ct);
+      Frame<R> Pe;
+      {
+        const R ctc = ct * cos_phi, cts = ct * sin_phi, stc = st * cos_phi, sts = st * sin_phi;
+        Pe.m = {ctc * Pc.m.x + cts * Pc.n.x - st * Pc.s.x, ctc * Pc.m.y + cts * Pc.n.y - st * Pc.s.y,
+                ctc * Pc.m.z + cts * Pc.n.z - st * Pc.s.z};
+        Pe.n = {-sin_phi * Pc.m.x + cos_phi * Pc.n.x, -sin_phi * Pc.m.y + cos_phi * Pc.n.y,
+                -sin_phi * Pc.m.z + cos_phi * Pc.n.z};
+        Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
+                stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
+      }
+      const R fi = amp / M<R>::sqrt(F);
+      const cx<R> um = cadd(cscale(Em, cos_phi), cscale(En, sin_phi));
+      const cx<R> un = cadd(cscale(Em, -sin_phi), cscale(En, cos_phi));
+      const cx<R> fm = cscale(cmul(s2, um), fi), fn = cscale(cmul(s1, un), fi);
+      const cx<R> Efx = cadd(cscale(fm, Pe.m.x), cscale(fn, Pe.n.x));
+      const cx<R> Efy = cadd(cscale(fm, Pe.m.y), cscale(fn, Pe.n.y));
+      cx<R> Erx{R(0), R(0)}, Ery{R(0), R(0)};
+      if (!vsingle) {
+        cx<R> rm, rn;
+        reverse_suffix(v.stab, S1, P1, Pe, xm, xn, rm, rn);
+        // CBS path phase exp(i k (s_out + s_in) . (r_vertex - r_1))
+        const R arg = lc.k0 * ((so.x + s_in.x) * (pos.x - r1.x) + (so.y + s_in.y) * (pos.y - r1.y) +
+                               (so.z + s_in.z) * (pos.z - r1.z));
+        R sa, ca;
+        M<R>::sincos(arg, &sa, &ca);
+        const cx<R> pp{ca * amp, sa * amp};
+        rm = cmul(rm, pp);
+        rn = cmul(rn, pp);
+        Erx = cadd(cscale(rm, Pe.m.x), cscale(rn, Pe.n.x));
+        Ery = cadd(cscale(rm, Pe.m.y), cscale(rn, Pe.n.y));
+      }
+      const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
+      const int64_t off = t_idx * tplane + b;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
-      acc_add(lg, hg, in_lds, q * plane + off, static_cast<double>(val));
+      for (int q = 0; q < 8; ++q) {
+        const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
+        acc_add(lg, hg, in_lds, q * plane + off, static_cast<double>(val));
+      }
     }
   }
-  const uint64_t done = __ballot(1);  // hits += 1 per estimated vertex
-  if (__lane_id() == __ffsll(static_cast<unsigned long long>(done)) - 1)
-    atomicAdd(&v.lhits[si], static_cast<unsigned>(__popcll(done)));
 }
 
 // Laser::emit_photon (laser.cpp:77-99) and sample_emission_time (:42-75).
@@ -934,112 +976,121 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
-template <class R, bool kXo, bool kTrack, bool kAdj, bool kEst>
+template <class R, bool kXo, bool kTrack, bool kAdj, int kPart>
 __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
-  R u_fp;
-  if constexpr (kXo) {
-    u_fp = static_cast<R>(ph.xo.s.uniform());
+  // kPart 0: the whole event; 1: the flight up to the species of the next
+  // vertex (left in ph.cur_sp); 2: the scatter at that vertex.  Estimator
+  // runs split the event so the whole wave can work on each vertex in between.
+  int sp;
+  if constexpr (kPart == 2) {
+    sp = ph.cur_sp;
   } else {
-    uint4 b;
-    b.x = ph.rng.next();
-    b.y = ph.rng.next();
-    b.z = ph.rng.next();
-    b.w = ph.rng.next();
-    u_fp = u01<R>(b.x);
-    ph.w_th = (b.y & 0xFFFFFF00u) | (b.x & 0xFFu);  // theta: high 24 bits; low byte: aux bits 16-23
-    ph.w_ph = b.z;                                   // azimuth component: bits 8-31; low byte: aux bits 0-7
-    ph.w_az = (b.w & 0xFFFFFF00u) | (b.y & 0xFFu);  // azimuth angle: bits 8-31; low byte: aux bits 8-15
-  }
-  R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
-  const V3<R> dir = ph.P.s;
-  R new_z = ph.pos.z + dir.z * step;
-  // interface crossings (simulation.cpp:317-383), with the layer index tracked
-  // explicitly (the reference nudges z by 1e-12 and re-derives it)
-  for (;;) {
-    R zi = R(0);
-    bool has = false;
-    if (new_z > ph.pos.z) {
-      if (v.L[ph.layer].nbr & 1) {
-        zi = v.L[ph.layer].z_max;
-        has = zi <= new_z;
+    R u_fp;
+    if constexpr (kXo) {
+      u_fp = static_cast<R>(ph.xo.s.uniform());
+    } else {
+      uint4 b;
+      b.x = ph.rng.next();
+      b.y = ph.rng.next();
+      b.z = ph.rng.next();
+      b.w = ph.rng.next();
+      u_fp = u01<R>(b.x);
+      ph.w_th = (b.y & 0xFFFFFF00u) | (b.x & 0xFFu);  // theta: high 24 bits; low byte: aux bits 16-23
+      ph.w_ph = b.z;                                   // azimuth component: bits 8-31; low byte: aux bits 0-7
+      ph.w_az = (b.w & 0xFFFFFF00u) | (b.y & 0xFFu);  // azimuth angle: bits 8-31; low byte: aux bits 8-15
+    }
+    R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
+    const V3<R> dir = ph.P.s;
+    R new_z = ph.pos.z + dir.z * step;
+    // interface crossings (simulation.cpp:317-383), with the layer index tracked
+    // explicitly (the reference nudges z by 1e-12 and re-derives it)
+    for (;;) {
+      R zi = R(0);
+      bool has = false;
+      if (new_z > ph.pos.z) {
+        if (v.L[ph.layer].nbr & 1) {
+          zi = v.L[ph.layer].z_max;
+          has = zi <= new_z;
+        }
+      } else if (new_z < ph.pos.z) {
+        if (v.L[ph.layer].nbr & 2) {
+          zi = v.L[ph.layer].z_min;
+          has = zi >= new_z;
+        }
       }
-    } else if (new_z < ph.pos.z) {
-      if (v.L[ph.layer].nbr & 2) {
-        zi = v.L[ph.layer].z_min;
-        has = zi >= new_z;
-      }
-    }
-    if (!has) break;
-    const R dzt = dir.z * step;
-    if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) break;
-    const R t = (zi - ph.pos.z) / dzt;
-    if (t < R(0) || t > R(1)) break;
-    const R step_to = step * t;
-    const R rem = (step - step_to) * v.L[ph.layer].mu_t;
-    ph.gpl += step_to;
-    ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step_to);
-    add_path<R, kXo, kAdj>(ph, step_to);
-    const V3<R> prev = ph.pos;
-    ph.pos.x += dir.x * step_to;
-    ph.pos.y += dir.y * step_to;
-    const bool up = dir.z > R(0);
-    if constexpr (sizeof(R) == 8)
-      ph.pos.z = zi + (up ? 1e-12 : -1e-12);
-    else
-      ph.pos.z = zi;
-    if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
-      ph.alive = false;
-      return 0;
-    }
-    ph.layer += up ? 1 : -1;
-    const R mt = v.L[ph.layer].mu_t;
-    step = mt > R(0) ? rem / mt : rem;
-    new_z = ph.pos.z + dir.z * step;
-  }
-  ph.gpl += step;
-  ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step);
-  add_path<R, kXo, kAdj>(ph, step);
-  {
-    const V3<R> prev = ph.pos;
-    ph.pos.x += dir.x * step;
-    ph.pos.y += dir.y * step;
-    ph.pos.z += dir.z * step;
-    if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
-      ph.alive = false;
-      return 0;
-    }
-  }
-  // Sample::is_inside (sample.cpp:268-288)
-  if (!(ph.pos.z >= R(0)) || ph.pos.z >= v.L[ph.layer].z_top || !isfinite(ph.pos.x) ||
-      !isfinite(ph.pos.y) || !isfinite(ph.pos.z)) {
-    ph.alive = false;
-    return 0;
-  }
-  // species of this vertex (MixtureLayer::select_scatter_medium, sample.cpp:96-104)
-  const LayerDev<R>& Ld = v.L[ph.layer];
-  int sp = Ld.sp_first;
-  if (Ld.sp_count > 1) {
-    R u;
-    if constexpr (kXo)
-      u = static_cast<R>(ph.xo.s.uniform());
-    else
-      u = aux_uniform<R>(ph);
-    const R* c = v.sel + Ld.sel_off;
-    int lo = 0, hi = Ld.sp_count;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (c[mid] < u)
-        lo = mid + 1;
+      if (!has) break;
+      const R dzt = dir.z * step;
+      if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) break;
+      const R t = (zi - ph.pos.z) / dzt;
+      if (t < R(0) || t > R(1)) break;
+      const R step_to = step * t;
+      const R rem = (step - step_to) * v.L[ph.layer].mu_t;
+      ph.gpl += step_to;
+      ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step_to);
+      add_path<R, kXo, kAdj>(ph, step_to);
+      const V3<R> prev = ph.pos;
+      ph.pos.x += dir.x * step_to;
+      ph.pos.y += dir.y * step_to;
+      const bool up = dir.z > R(0);
+      if constexpr (sizeof(R) == 8)
+        ph.pos.z = zi + (up ? 1e-12 : -1e-12);
       else
-        hi = mid;
+        ph.pos.z = zi;
+      if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+        ph.alive = false;
+        return 0;
+      }
+      ph.layer += up ? 1 : -1;
+      const R mt = v.L[ph.layer].mu_t;
+      step = mt > R(0) ? rem / mt : rem;
+      new_z = ph.pos.z + dir.z * step;
     }
-    sp += lo < Ld.sp_count ? lo : Ld.sp_count - 1;
+    ph.gpl += step;
+    ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step);
+    add_path<R, kXo, kAdj>(ph, step);
+    {
+      const V3<R> prev = ph.pos;
+      ph.pos.x += dir.x * step;
+      ph.pos.y += dir.y * step;
+      ph.pos.z += dir.z * step;
+      if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+        ph.alive = false;
+        return 0;
+      }
+    }
+    // Sample::is_inside (sample.cpp:268-288)
+    if (!(ph.pos.z >= R(0)) || ph.pos.z >= v.L[ph.layer].z_top || !isfinite(ph.pos.x) ||
+        !isfinite(ph.pos.y) || !isfinite(ph.pos.z)) {
+      ph.alive = false;
+      return 0;
+    }
+    // species of this vertex (MixtureLayer::select_scatter_medium, sample.cpp:96-104)
+    const LayerDev<R>& Lv = v.L[ph.layer];
+    sp = Lv.sp_first;
+    if (Lv.sp_count > 1) {
+      R u;
+      if constexpr (kXo)
+        u = static_cast<R>(ph.xo.s.uniform());
+      else
+        u = aux_uniform<R>(ph);
+      const R* c = v.sel + Lv.sel_off;
+      int lo = 0, hi = Lv.sp_count;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (c[mid] < u)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      sp += lo < Lv.sp_count ? lo : Lv.sp_count - 1;
+    }
+    if constexpr (kPart == 1) {
+      ph.cur_sp = sp;
+      return 0;
+    }
   }
+  const LayerDev<R>& Ld = v.L[ph.layer];
   const SpeciesDev<R>& S = v.S[sp];
-  if constexpr (kEst) {  // SensorsGroup::run_estimators (detector.cpp:272-281)
-    for (int si = 0; si < p.n_sensors; ++si)
-      if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_vertex<R, kXo>(p, v, ph, si, S, Ld.albedo);
-  }
   // theta (medium.cpp:51-58 -> phase.cpp)
   R uth;
   if constexpr (kXo)
@@ -1373,8 +1424,14 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
       continue;
     }
     int32_t ev0 = ph.events;
-    if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj, kEst>(p, v, ph);
+    if constexpr (kEst) {
+      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1>(p, v, ph);
+      // SensorsGroup::run_estimators (detector.cpp:272-281) with the whole wave
+      for (int si = 0; si < p.n_sensors; ++si)
+        if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_wave<R, kXo>(p, v, ph, si);
+      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2>(p, v, ph);
+    } else if (ph.alive) {
+      const int tr = transport_event<R, kXo, kTrack, kAdj, 0>(p, v, ph);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));

@@ -227,9 +227,13 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.cdf_step = static_cast<R>(d.cdf_step);
     o.hg_g = static_cast<R>(d.hg_g);
     if (d.phase_kind == LMC_PHASE_TABLE_THETA || d.phase_kind == LMC_PHASE_TABLE_COS) {
-      if (d.cdf_n < 2 || d.cdf_n > 65535) return fail(LMC_ERR_INVALID, "cdf_n must be in [2, 65535]");
+      if (d.cdf_n < 2 || d.cdf_n > (1 << 24)) return fail(LMC_ERR_INVALID, "cdf_n must be in [2, 2^24]");
       o.cdf_n = d.cdf_n;
       o.guide_n = kGuideN;
+      // u16 guide entries: indices of tables beyond 65536 nodes are stored
+      // shifted right, and the search widens each bracket by 2^shift - 1
+      o.guide_shift = 0;
+      while (((d.cdf_n - 1) >> o.guide_shift) > 65535) ++o.guide_shift;
       const double* c = s->tables.data() + d.cdf_offset;
       std::vector<double> key(c, c + d.cdf_n);
       auto hit = cdf_seen.find(key);
@@ -248,9 +252,9 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
         for (int j = 0; j < kGuideN; ++j) {
           const R thr = static_cast<R>(static_cast<double>(j) / kGuideN);
           while (idx < d.cdf_n - 1 && cr[idx] < thr) ++idx;
-          guide.push_back(static_cast<uint16_t>(idx));
+          guide.push_back(static_cast<uint16_t>(idx >> o.guide_shift));
         }
-        guide.push_back(static_cast<uint16_t>(d.cdf_n - 1));
+        guide.push_back(static_cast<uint16_t>((d.cdf_n - 1) >> o.guide_shift));
       }
     }
     if (d.s_n < 2) return fail(LMC_ERR_INVALID, "species S table needs >= 2 nodes");

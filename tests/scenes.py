@@ -151,3 +151,27 @@ def scene_polydisperse(K: int = 32, n_table: int = 2048, n_div: int = 10000, n_t
                                                      field_len / field_n, absorb=False))
     keep += [layer, cbs, field]
     return sample, laser, group.detectors, keep
+
+
+def scene_estimator(n_theta: int = 300):
+    """The reference's estimator study (results/estimator_vs_full/sim.py) in mm:
+    semi-infinite non-absorbing medium of RGD spheres (r = 0.110 um, n_p = 1.59
+    in n = 1.33, lambda = 0.514 um, RDG-EMC theta sampler on 100000 divisions),
+    mu_s 10 /mm; circularly polarised Gaussian beam (radius 4 mean free paths);
+    one FarFieldCBSSensor in estimator mode, 10 degrees in n_theta x 1 bins,
+    time-integrated."""
+    n = 1.33
+    wl, r = 0.514e-3, 0.110e-3
+    ph = lmc.RayleighDebyeEMCPhaseFunction(wl, r, 1.59, n, 100000, 0.0, math.pi)
+    m = lmc.RGDMedium(ph, r, 1.59, n, wl)
+    mu_s = 10.0
+    m.set_scattering_coefficient(mu_s)
+    m.set_absorption_coefficient(0.0)
+    m.set_mean_free_path(1.0 / mu_s)
+    sample = lmc.Sample(n)
+    sample.add_layer(m, 0.0, math.inf)
+    laser = lmc.Laser(1 / math.sqrt(2), -1j / math.sqrt(2), wl, 4.0 / mu_s, lmc.LaserSource.Gaussian)
+    group = lmc.SensorsGroup()
+    th = math.radians(10.0)
+    group.add_detector(lmc.FarFieldCBSSensor(th, 2 * math.pi, 0.0, th / n_theta, 2 * math.pi, 0.0, True))
+    return sample, laser, group.detectors, (m, ph)

@@ -125,6 +125,15 @@ def test_gpu_fp64_xoshiro_estimator_phi_slices(require_gpu):
     assert np.abs(g[0, 0, :5]).min() > 0
 
 
+def test_gpu_fp64_xoshiro_estimator_study_scene(require_gpu):
+    """The reference's estimator study (300 theta bins, RDG-EMC table on 100001
+    nodes: a guide table whose entries are stored shifted)."""
+    sample, laser, sensors, keep = scenes.scene_estimator()
+    cfg = _cfg(sample, laser, sensors, keep)
+    cfg.seed = 24
+    _fp64_parity(cfg, 300, 24)
+
+
 def test_gpu_fp32_estimator_statistical_parity(require_gpu):
     N, B, seed = 32000, 16, 31
     cfg = _mie_estimator(time_resolved=False)
