@@ -510,7 +510,9 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.key0 = static_cast<uint32_t>(r->seed);
   p.key1 = static_cast<uint32_t>(r->seed >> 32);
   p.max_events = r->max_events;
-  p.claim = r->claim > 0 ? r->claim : 256;
+  // photons a wave claims at a time; an estimator photon costs hundreds of
+  // plain ones, so its claims are small enough to keep the launch's tail short
+  p.claim = r->claim > 0 ? r->claim : (s->has_est ? 8 : 256);
   p.out = s->out.p;
   p.hits = s->hits.p;
   p.counter = s->counter.p;
