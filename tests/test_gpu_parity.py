@@ -249,3 +249,21 @@ def test_chunked_runs_equal_one_run(require_gpu):
     two, h2 = sc.read()
     assert int(h1[0]) == int(h2[0])
     assert np.allclose(one, two, rtol=1e-12, atol=1e-15)
+
+
+def test_gpu_fp64_xoshiro_parity_steady_fluence_first(require_gpu):
+    """A steady-state planar fluence as the first sensor (accumulators at offset
+    0: its grid row must be row 0, nothing before it) beside an analog CBS."""
+    cfg, _ = scenes.config_plain_mie(1, seed=13)
+    group = lmc.SensorsGroup()
+    group.add_detector(lmc.PlanarFluenceSensor(0.0, 4.0, 4.0, 0.0, 0.25, 0.25, 0.0, absorb=False))
+    group.add_detector(lmc.FarFieldCBSSensor(0.1, 6.28, 0.0, 0.1 / 20, 6.28, 0.0))
+    cfg.detector = group
+    sc = _scene(cfg)
+    sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
+    gpu, hits = sc.read()
+    cpu, chits, _, _ = oracle.run(sc.build, 20000, 13)
+    print("hits", list(hits), list(chits))
+    assert np.abs(cpu[:sc.build.offsets[0][1]]).max() > 0
+    _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
+    sc.close()
