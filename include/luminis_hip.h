@@ -93,8 +93,9 @@ enum {
   LMC_SENSOR_PLANAR_FIELD = 3,   /* PlanarFieldSensor                          */
   LMC_SENSOR_FARFIELD_FLUENCE = 4, /* extension: exit-direction Stokes histogram
                                       (no reference counterpart)             */
-  LMC_SENSOR_REFLECTANCE = 5     /* extension: sum of exit weights (the
+  LMC_SENSOR_REFLECTANCE = 5,    /* extension: sum of exit weights (the
                                     PhotonRecordSensor weight sum)          */
+  LMC_SENSOR_STATISTICS = 6      /* StatisticsSensor (detector.cpp:1571-1888) */
 };
 
 /* Crossing-direction filter values (detector.hpp CrossingDirection). */
@@ -111,7 +112,17 @@ enum { LMC_DIR_FORWARD = 0, LMC_DIR_BACKWARD = 1, LMC_DIR_BOTH = 2 };
  *             Ey grid, [x][y] complex (re, im) doubles.
  *  FF_FLUENCE g = theta_max, phi_max, dtheta, dphi; n = N_theta, N_phi.
  *             Output: 4 grids [S0..S3][theta][phi].
- *  REFLECTANCE: output 1 double (sum of weights). */
+ *  REFLECTANCE: output 1 double (sum of weights).
+ *  STATISTICS: the histogram parameters are 27 doubles at tables[stat_offset]:
+ *             [0..5] set flags of the events, theta, phi, depth, time and
+ *             weight histograms; [6] max_events; [7..10] min_theta,
+ *             max_theta, n_bins_theta, dtheta; [11..14] the same for phi;
+ *             [15..17] max_depth, n_bins_depth, ddepth; [18..20] max_time,
+ *             n_bins_time, dtime; [21..23] max_weight, n_bins_weight,
+ *             dweight; [24..26] N_t, dt, t_max (set_time_resolution).
+ *             Output: the set histograms in that order, counts as doubles:
+ *             events [N_t][max_events], theta [N_t][n], phi [N_t][n],
+ *             depth [N_t][n], time [n], weight [n]. */
 typedef struct lmc_sensor_desc {
   int32_t kind;
   int32_t absorb;
@@ -139,6 +150,7 @@ typedef struct lmc_sensor_desc {
                              detector.cpp:909-943); 0: (j + 1/2) * d_phi    */
   int64_t phi_offset;
   double theta_pp_max;  /* estimator theta cap when > 0 (detector.cpp:1382)   */
+  int64_t stat_offset;  /* STATISTICS: parameter block in tables (above)      */
 } lmc_sensor_desc;
 
 /* Laser (include/luminis/core/laser.hpp:45-90, src/laser.cpp:29-99). */

@@ -320,9 +320,10 @@ struct SensorDev {
   int32_t lds_off, pad2;
   // estimator mode: est != 0 takes no crossings; a FARFIELD_CBS sensor then
   // deposits every vertex's forced scatter into its first est_imax theta rows
-  // (bin geometry at est_geom in the geometry pool), in the transport
-  // kernel's LDS copy at est_lds (doubles) or, when est_lds < 0, in HBM
-  int32_t est, est_imax, est_geom, est_lds;
+  // (bin geometry at aux in the auxiliary pool), in the transport kernel's
+  // LDS copy at est_lds (doubles) or, when est_lds < 0, in HBM.  A STATISTICS
+  // sensor keeps its 27 histogram parameters at aux.
+  int32_t est, est_imax, aux, est_lds;
   // conservative grid bounds for the transport kernel's queueing test: exit
   // cosine -s_z >= q_cmin, hit point in [q_x0, q_x1] x [q_y0, q_y1], arrival
   // in [q_t0, q_t1] (slightly wider than the grid; the sensor kernel bins exactly)

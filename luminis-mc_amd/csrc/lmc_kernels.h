@@ -60,7 +60,7 @@ template <class R> struct KParams {
   const PlaneDev* planes;
   const LaunchConst<R>* lcs;     // one per scene
   const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
-  const R* est_geom;             // estimator bin geometry pool
+  const R* aux;                  // estimator bin geometry and statistics histogram parameters
   int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
 };
 
@@ -100,7 +100,7 @@ template <class R> __device__ __forceinline__ Quad<R> get4(const R* f, int q) {
     return {u.x, u.y, v.x, v.y};
   }
 }
-enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, kRecI = 4 };
+enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, I_DEPTH = 4, kRecI = 5 };  // I_DEPTH: float bits
 
 template <class R> struct SceneView {
   const LaunchConst<R>* lc;
@@ -132,6 +132,7 @@ template <class R, bool kXo> struct Photon {
   Frame<R> P;
   cx<R> Em, En;
   R w, gpl, t0;
+  R depth;  // maximum z reached (penetration depth, simulation.cpp:396-400)
   double opl;
   int32_t events, layer;
   int32_t cur_sp;    // species of the pending vertex between the two parts of an estimator event
@@ -446,6 +447,66 @@ template <class R> __device__ __forceinline__ R stokes(int q, cx<R> Ex, cx<R> Ey
   return R(2) * (Ex.i * Ey.r - Ex.r * Ey.i);
 }
 
+// StatisticsSensor::process_hit (detector.cpp:1703-1800): one count in each
+// set histogram (events, exit theta and phi, penetration depth, each also in
+// its time window; arrival time; weight).  Parameters at p.aux + sd.aux in the
+// order of the scene descriptor's block; an index the reference could compute
+// one past the end (a value just below the upper edge) is clamped.
+template <class R, bool kAdj>
+__device__ __forceinline__ void stats_deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int events,
+                                              V3<R> dir, R arr, R w, float depth, const R* score) {
+  const R* a = p.aux + sd.aux;
+  const int Nt = static_cast<int>(a[24]);
+  const R dt = a[25], tmax = a[26];
+  int t_idx = -1;
+  if (Nt > 1 && dt > R(0) && arr >= R(0) && arr < tmax) {
+    t_idx = static_cast<int>(arr / dt) + 1;
+    if (t_idx >= Nt) t_idx = -1;
+  }
+  int64_t base = 0;
+  auto temporal = [&](int n, int idx) {
+    idx = idx < n - 1 ? idx : n - 1;
+    deposit<R, kAdj>(p, sd, lds_grid, base + idx, 1.0, score);
+    if (t_idx >= 1) deposit<R, kAdj>(p, sd, lds_grid, base + static_cast<int64_t>(t_idx) * n + idx, 1.0, score);
+  };
+  if (a[0] != R(0)) {
+    const int me = static_cast<int>(a[6]);
+    if (events >= 0 && events < me) temporal(me, events);
+    base += static_cast<int64_t>(Nt) * me;
+  }
+  if (a[1] != R(0)) {
+    const int n = static_cast<int>(a[9]);
+    const R theta = exit_theta(dir);
+    if (theta >= a[7] && theta < a[8]) temporal(n, static_cast<int>((theta - a[7]) / a[10]));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[2] != R(0)) {
+    const int n = static_cast<int>(a[13]);
+    const R phi = exit_phi(dir);
+    if (phi >= a[11] && phi < a[12]) temporal(n, static_cast<int>((phi - a[11]) / a[14]));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[3] != R(0)) {
+    const int n = static_cast<int>(a[16]);
+    const R d = static_cast<R>(depth);
+    if (d >= R(0) && d < a[15]) temporal(n, static_cast<int>(d / a[17]));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[4] != R(0)) {
+    const int n = static_cast<int>(a[19]);
+    const int idx = static_cast<int>(arr / a[20]);
+    if (idx >= 1 && idx < n) deposit<R, kAdj>(p, sd, lds_grid, base + idx, 1.0, score);
+    base += n;
+  }
+  if (a[5] != R(0)) {
+    const int n = static_cast<int>(a[22]);
+    if (w >= R(0) && w < a[21]) {
+      const int idx = static_cast<int>(w / a[23]);
+      deposit<R, kAdj>(p, sd, lds_grid, base + (idx < n - 1 ? idx : n - 1), 1.0, score);
+    }
+  }
+}
+
 // Deposit the 8 CBS grids for one detection: coherent |E_f + E_r|^2 terms and
 // incoherent |E_f|^2 + |E_r|^2 terms (accumulate_stokes, detector.cpp:1194-1226).
 // A time-resolved hit goes to its time bin only; the reference's second copy
@@ -624,6 +685,7 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   iq[I_MASK * c] = mask;
   iq[I_EVENTS * c] = ph.events;
   iq[I_SP * c] = ph.sp;
+  iq[I_DEPTH * c] = __float_as_int(static_cast<float>(ph.depth));
   return killed;
 }
 
@@ -744,7 +806,7 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
   const V3<R> e1{lc.lm[0], lc.lm[1], lc.lm[2]}, e2{lc.ln[0], lc.ln[1], lc.ln[2]};
   const V3<R> s_in{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
   const int Nth = sd.n0, Nph = sd.n1;
-  const R* gc = p.est_geom + sd.est_geom;  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each)
+  const R* gc = p.aux + sd.aux;  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each)
   const R dphi = static_cast<R>(sd.g[4]);
   const R tmax = static_cast<R>(sd.g[2]), dt = static_cast<R>(sd.g[5]);
   const R zp = static_cast<R>(v.planes[sd.plane].z);
@@ -937,6 +999,7 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
   ph.Em = {L.pol[0], L.pol[1]};
   ph.En = {L.pol[2], L.pol[3]};
   ph.w = R(1);
+  ph.depth = R(0);
   ph.gpl = R(0);
   ph.t0 = t0;
   ph.opl = 0.0;
@@ -1053,6 +1116,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       ph.pos.x += dir.x * step;
       ph.pos.y += dir.y * step;
       ph.pos.z += dir.z * step;
+      if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
       if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
         ph.alive = false;
         return 0;
@@ -1543,6 +1607,11 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
       if (!((mask >> k) & 1)) continue;
       const int si = first + k;
       const SensorDev& sd = p.sensors[si];
+      if (sd.kind == LMC_SENSOR_STATISTICS) {
+        stats_deposit<R, kAdj>(p, sd, sgrid, events, P.s, arr, w, __int_as_float(iq[I_DEPTH * c]), score);
+        atomicAdd(&shits[si], 1u);
+        continue;
+      }
       int64_t bin;
       int t_idx;
       if (!sensor_bin(sd, hx, hy, arr, P.s, bin, t_idx)) continue;
@@ -1611,7 +1680,10 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
     if (!sd.lds) continue;
     const int64_t len = (sd.kind == LMC_SENSOR_FARFIELD_CBS)
                             ? 8ll * sd.n0 * sd.n1 * sd.n2
-                            : (sd.kind == LMC_SENSOR_REFLECTANCE ? 1ll : 4ll * sd.n0 * sd.n1);
+                            : (sd.kind == LMC_SENSOR_REFLECTANCE
+                                   ? 1ll
+                                   : (sd.kind == LMC_SENSOR_STATISTICS ? static_cast<int64_t>(sd.n0)
+                                                                       : 4ll * sd.n0 * sd.n1));
     for (int64_t i = tid; i < len; i += nt) {
       const double val = sgrid[sd.lds_off + i];
       if (val != 0.0) unsafeAtomicAdd(p.out + sd.out_off + i, val);

@@ -107,7 +107,7 @@ template <class R> struct DevTables {
   DBuf<uint16_t> guide;
   DBuf<STab<R>> stab;
   DBuf<LaunchConst<R>> lcs;
-  DBuf<R> est_geom;
+  DBuf<R> aux;
   int cdf_total = 0, guide_total = 0, stab_total = 0, sel_total = 0;
   size_t table_bytes = 0;
 };
@@ -149,7 +149,7 @@ struct lmc_scene {
   int lds_grid = 0;
   // estimator-mode CBS sensors: bin geometry (double, converted per precision)
   // and the doubles of their grids staged in the transport kernel's LDS
-  std::vector<double> est_geom_host;
+  std::vector<double> aux_host;
   int est_lds_n = 0;
   bool has_est = false;
   DBuf<double> out;
@@ -342,9 +342,9 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if ((rc = T.species.upload(sp.data(), sp.size()))) return rc;
   if ((rc = T.lcs.upload(lc.data(), lc.size()))) return rc;
   {
-    std::vector<R> eg(s->est_geom_host.begin(), s->est_geom_host.end());
+    std::vector<R> eg(s->aux_host.begin(), s->aux_host.end());
     if (eg.empty()) eg.push_back(R(0));
-    if ((rc = T.est_geom.upload(eg.data(), eg.size()))) return rc;
+    if ((rc = T.aux.upload(eg.data(), eg.size()))) return rc;
   }
   if (cdf.empty()) cdf.push_back(R(1));
   if (guide.empty()) guide.assign(2, 0);
@@ -406,6 +406,20 @@ int validate(const lmc_scene_desc* d) {
       case LMC_SENSOR_PLANAR_FIELD: need = 4ll * s.n[0] * s.n[1]; break;
       case LMC_SENSOR_FARFIELD_FLUENCE: need = 4ll * s.n[0] * s.n[1]; break;
       case LMC_SENSOR_REFLECTANCE: need = 1; break;
+      case LMC_SENSOR_STATISTICS: {
+        if (s.stat_offset < 0 || s.stat_offset + 27 > d->n_tables)
+          return fail(LMC_ERR_INVALID, "statistics parameters out of bounds");
+        const double* a = d->tables + s.stat_offset;
+        const int64_t nt = static_cast<int64_t>(a[24]);
+        const double sizes[6] = {nt * a[6], nt * a[9], nt * a[13], nt * a[16], a[19], a[22]};
+        for (int k = 0; k < 6; ++k) {
+          if (a[k] == 0.0) continue;
+          if (!(sizes[k] >= 1) || sizes[k] > 1e9) return fail(LMC_ERR_INVALID, "statistics histogram size");
+          need += static_cast<int64_t>(sizes[k]);
+        }
+        if (nt < 1) return fail(LMC_ERR_INVALID, "statistics N_t must be >= 1");
+        break;
+      }
       default: return fail(LMC_ERR_INVALID, "unknown sensor kind");
     }
     if (need != s.out_len) return fail(LMC_ERR_INVALID, "sensor out_len does not match its grid");
@@ -497,6 +511,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.sensors = s->sensors_dev.p;
   p.planes = s->planes_dev.p;
   p.lcs = T.lcs.p;
+  p.aux = T.aux.p;  // estimator bin geometry, statistics histogram parameters
   p.n_layers = static_cast<int32_t>(s->layers.size());
   p.n_species = static_cast<int32_t>(s->species.size());
   p.n_sensors = static_cast<int32_t>(s->sensors_host.size());
@@ -593,7 +608,6 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     int per_cu = 0;
     LMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kt_est, block, lds));
     bpc = std::max(1, per_cu);
-    p.est_geom = T.est_geom.p;
     p.est_lds_n = s->est_lds_n;
   }
   const int grid = g_num_cu * bpc;
@@ -929,6 +943,12 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       o.n2 = sd.n[2];
       o.out_off = sc.out_base + sd.out_offset;
       queue_bounds(o);
+      if (sd.kind == LMC_SENSOR_STATISTICS) {  // histogram parameters; n0 = accumulator length
+        o.aux = static_cast<int32_t>(s->aux_host.size());
+        s->aux_host.insert(s->aux_host.end(), d->tables + sd.stat_offset, d->tables + sd.stat_offset + 27);
+        o.n0 = static_cast<int32_t>(sd.out_len);
+        o.n1 = o.n2 = 0;
+      }
       o.est = sd.estimator ? 1 : 0;
       o.est_lds = -1;
       if (o.est) {
@@ -941,8 +961,8 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
           const double theta_max = sd.g[0], dth = sd.g[3], dph = sd.g[4];
           const double th_cap = sd.theta_pp_max > 0.0 ? std::min(sd.theta_pp_max, theta_max) : theta_max;
           o.est_imax = std::min(sd.n[0], static_cast<int>(th_cap / dth));
-          o.est_geom = static_cast<int32_t>(s->est_geom_host.size());
-          auto& eg = s->est_geom_host;
+          o.aux = static_cast<int32_t>(s->aux_host.size());
+          auto& eg = s->aux_host;
           for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::cos((it + 0.5) * dth));
           for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::sin((it + 0.5) * dth));
           for (int it = 0; it < sd.n[0]; ++it) eg.push_back(std::cos(it * dth) - std::cos((it + 1) * dth));
@@ -965,7 +985,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       // privatise small angular grids per workgroup of the sensor kernel (<= 48 KiB)
       const bool small = privatise && !o.est &&
                          (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_FARFIELD_FLUENCE ||
-                          sd.kind == LMC_SENSOR_REFLECTANCE) &&
+                          sd.kind == LMC_SENSOR_REFLECTANCE || sd.kind == LMC_SENSOR_STATISTICS) &&
                          (s->lds_grid + sd.out_len) * 8 <= 49152;
       if (small) {
         o.lds = 1;

@@ -15,6 +15,7 @@ from .phase import (DrainePhaseFunction, HenyeyGreensteinPhaseFunction, MiePhase
 from .medium import Layer, MieMedium, MixtureLayer, RGDMedium, Sample, SampleLayer, ScatteringMedium
 from .sensors import (CrossingDirection, FarFieldCBSSensor, FarFieldFluenceSensor, Laser, LaserSource,
                       PlanarFieldSensor, PlanarFluenceSensor, ReflectanceSensor, Sensor, SensorsGroup,
+                      StatisticsSensor,
                       TemporalProfile)
 from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFieldProcessed,
                       PlanarFluenceProcessed, StokesMatrixProcessed, StokesRadialProcessed,

@@ -34,6 +34,7 @@ SENSOR_PLANAR_FLUENCE = 2
 SENSOR_PLANAR_FIELD = 3
 SENSOR_FARFIELD_FLUENCE = 4
 SENSOR_REFLECTANCE = 5
+SENSOR_STATISTICS = 6
 
 PREC_FP32 = 0
 PREC_FP64 = 1
@@ -71,6 +72,7 @@ class SensorDesc(ctypes.Structure):
         ("g", c_double * 8), ("n", c_int32 * 4),
         ("out_offset", c_int64), ("out_len", c_int64),
         ("estimator", c_int32), ("n_phi_explicit", c_int32), ("phi_offset", c_int64), ("theta_pp_max", c_double),
+        ("stat_offset", c_int64),
     ]
 
 

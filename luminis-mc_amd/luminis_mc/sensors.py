@@ -313,6 +313,114 @@ class ReflectanceSensor(Sensor):
         return float(self.buffer[0])
 
 
+class StatisticsSensor(Sensor):
+    """Histograms of the photons crossing its plane: scattering events, exit
+    polar and azimuthal angles, penetration depth, arrival time and weight
+    (detector.cpp:1571-1888); each histogram is off until its set_* call."""
+
+    kind = _lib.SENSOR_STATISTICS
+    _KEYS = ("events", "theta", "phi", "depth", "time", "weight")
+
+    def __init__(self, z: float, absorb: bool = False):
+        super().__init__(z, absorb, False)
+        for k in self._KEYS:
+            setattr(self, f"{k}_histogram_bins_set", False)
+        self.max_events = 0
+        self.min_theta = self.max_theta = self.dtheta = 0.0
+        self.n_bins_theta = 0
+        self.min_phi = self.max_phi = self.dphi = 0.0
+        self.n_bins_phi = 0
+        self.max_depth = self.ddepth = 0.0
+        self.n_bins_depth = 0
+        self.h_max_time = self.h_dtime = 0.0
+        self.n_bins_time = 0
+        self.max_weight = self.dweight = 0.0
+        self.n_bins_weight = 0
+        self.N_t, self.dt, self.t_max = 1, 0.0, 0.0
+        self._resize()
+
+    # -- histogram configuration (detector.cpp:1802-1888)
+    def set_time_resolution(self, len_t: float, dt: float) -> None:
+        self.t_max, self.dt = float(len_t), float(dt)
+        if self.dt <= 0.0:
+            self.N_t, self.dt = 1, 0.0
+        else:
+            self.N_t = int(math.ceil(len_t / dt)) + 1
+        self._resize()
+
+    def set_events_histogram_bins(self, max_events: int) -> None:
+        self.max_events = int(max_events)
+        self.events_histogram_bins_set = True
+        self._resize()
+
+    def set_theta_histogram_bins(self, min_theta: float, max_theta: float, n_bins: int) -> None:
+        self.min_theta, self.max_theta, self.n_bins_theta = float(min_theta), float(max_theta), int(n_bins)
+        self.dtheta = (self.max_theta - self.min_theta) / self.n_bins_theta
+        self.theta_histogram_bins_set = True
+        self._resize()
+
+    def set_phi_histogram_bins(self, min_phi: float, max_phi: float, n_bins: int) -> None:
+        self.min_phi, self.max_phi, self.n_bins_phi = float(min_phi), float(max_phi), int(n_bins)
+        self.dphi = (self.max_phi - self.min_phi) / self.n_bins_phi
+        self.phi_histogram_bins_set = True
+        self._resize()
+
+    def set_depth_histogram_bins(self, max_depth: float, n_bins: int) -> None:
+        self.max_depth, self.n_bins_depth = float(max_depth), int(n_bins)
+        self.ddepth = self.max_depth / self.n_bins_depth
+        self.depth_histogram_bins_set = True
+        self._resize()
+
+    def set_time_histogram_bins(self, max_time: float, n_bins: int) -> None:
+        self.h_max_time, self.n_bins_time = float(max_time), max(1, int(n_bins))
+        self.h_dtime = self.h_max_time / int(n_bins)
+        self.time_histogram_bins_set = True
+        self._resize()
+
+    def set_weight_histogram_bins(self, max_weight: float, n_bins: int) -> None:
+        self.max_weight, self.n_bins_weight = float(max_weight), int(n_bins)
+        self.dweight = self.max_weight / self.n_bins_weight
+        self.weight_histogram_bins_set = True
+        self._resize()
+
+    # -- accumulator layout
+    def _sizes(self):
+        nt = self.N_t
+        return [("events", nt, self.max_events), ("theta", nt, self.n_bins_theta), ("phi", nt, self.n_bins_phi),
+                ("depth", nt, self.n_bins_depth), ("time", 1, self.n_bins_time), ("weight", 1, self.n_bins_weight)]
+
+    def _resize(self) -> None:
+        n = sum(a * b for k, a, b in self._sizes() if getattr(self, f"{k}_histogram_bins_set"))
+        self.buffer = np.zeros(n)
+
+    def _block(self, key):
+        off = 0
+        for k, a, b in self._sizes():
+            if not getattr(self, f"{k}_histogram_bins_set"):
+                continue
+            if k == key:
+                blk = np.rint(self.buffer[off:off + a * b]).astype(np.int64).reshape(a, b)
+                return blk.tolist() if k in ("events", "theta", "phi", "depth") else blk[0].tolist()
+            off += a * b
+        return [] if key in ("time", "weight") else []
+
+    events_histogram = property(lambda self: self._block("events"))
+    theta_histogram = property(lambda self: self._block("theta"))
+    phi_histogram = property(lambda self: self._block("phi"))
+    depth_histogram = property(lambda self: self._block("depth"))
+    time_histogram = property(lambda self: self._block("time"))
+    weight_histogram = property(lambda self: self._block("weight"))
+
+    def stat_params(self) -> np.ndarray:
+        flags = [float(getattr(self, f"{k}_histogram_bins_set")) for k in self._KEYS]
+        return np.array(flags + [self.max_events, self.min_theta, self.max_theta, self.n_bins_theta, self.dtheta,
+                                 self.min_phi, self.max_phi, self.n_bins_phi, self.dphi,
+                                 self.max_depth, self.n_bins_depth, self.ddepth,
+                                 self.h_max_time, self.n_bins_time, self.h_dtime,
+                                 self.max_weight, self.n_bins_weight, self.dweight,
+                                 self.N_t, self.dt, self.t_max], dtype=np.float64)
+
+
 class SensorsGroup:
     """Container; add_detector stores a clone and returns the internal copy
     (python_module.cpp:315-322, detector.cpp:166-176)."""

@@ -144,6 +144,8 @@ class SceneBuild:
         for i, s in enumerate(sensors):
             d = se_arr[i]
             s._fill_desc(d)
+            if s.kind == _lib.SENSOR_STATISTICS:  # histogram parameter block
+                d.stat_offset = put(s.stat_params())
             if getattr(s, "phi_explicit", False):  # set_phi_slices columns
                 d.n_phi_explicit = len(s.phi_values)
                 d.phi_offset = put(np.asarray(s.phi_values, dtype=np.float64))

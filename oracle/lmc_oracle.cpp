@@ -18,6 +18,8 @@
 //                                accumulate_stokes, FarFieldCBSSensor::process_hit
 //   src/detector.cpp:36-161,1012-1044,1315-1566  the CBS estimator
 //                                (process_estimation and its helpers)
+//   src/detector.cpp:1703-1800   StatisticsSensor::process_hit (and the
+//                                penetration depth, simulation.cpp:396-400)
 //   src/sample.cpp:90-104,221-288 layer navigation, mixture species draw
 //   src/medium.cpp:123-126,175-191 free path, S(theta) lookup
 //   src/phase.cpp:27-49,108-115  azimuth rejection sampler, HG inversion
@@ -248,6 +250,7 @@ struct Layer {
 
 struct Sensor {
   lmc_sensor_desc d;
+  std::vector<double> stat;  // STATISTICS parameter block (27 values)
   // FarFieldCBS bin geometry (detector.cpp:884-907, 909-943)
   std::vector<double> cos_th, sin_th, domega, cos_ph, sin_ph;
 };
@@ -319,6 +322,7 @@ struct Photon {
   double launch_time{0};
   double velocity{1.0};
   double weight{1.0};
+  double penetration_depth{0.0};  // maximum z reached (simulation.cpp:396-400)
   size_t layer{0};
   int active{-1}, first_sp{-1}, last_sp{-1};
   M3 P;
@@ -760,6 +764,62 @@ void fffluence_process_hit(const lmc_sensor_desc& d, Photon& ph, const HitInfo& 
   hits += 1;
 }
 
+// StatisticsSensor::process_hit (detector.cpp:1703-1800): counts in the set
+// histograms; events, theta, phi and depth also per time window.
+void stats_process_hit(const Sensor& sen, Photon& ph, const HitInfo& info, double* g, uint64_t& hits) {
+  const std::vector<double>& a = sen.stat;
+  const int Nt = static_cast<int>(a[24]);
+  const double dt = a[25], tmax = a[26];
+  const double arrival = ph.launch_time + (info.opticalpath_at_hit / ph.velocity);
+  int t_idx = -1;
+  if (Nt > 1 && dt > 0.0 && arrival >= 0.0 && arrival < tmax) {
+    t_idx = static_cast<int>(arrival / dt) + 1;
+    if (t_idx >= Nt) t_idx = -1;
+  }
+  int64_t base = 0;
+  auto temporal = [&](int n, int idx) {
+    g[base + idx] += 1.0;
+    if (t_idx >= 1 && t_idx < Nt) g[base + static_cast<int64_t>(t_idx) * n + idx] += 1.0;
+  };
+  if (a[0] != 0.0) {
+    const int me = static_cast<int>(a[6]);
+    const int ev = static_cast<int>(ph.events);
+    if (ev >= 0 && ev < me) temporal(me, ev);
+    base += static_cast<int64_t>(Nt) * me;
+  }
+  if (a[1] != 0.0) {
+    const int n = static_cast<int>(a[9]);
+    const double theta = std::acos(-ph.P(2, 2));
+    if (theta >= a[7] && theta < a[8]) temporal(n, std::min(n - 1, static_cast<int>((theta - a[7]) / a[10])));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[2] != 0.0) {
+    const int n = static_cast<int>(a[13]);
+    double phi = std::atan2(ph.P(2, 1), ph.P(2, 0));
+    if (phi < 0) phi += 2.0 * kPi;
+    if (phi >= a[11] && phi < a[12]) temporal(n, std::min(n - 1, static_cast<int>((phi - a[11]) / a[14])));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[3] != 0.0) {
+    const int n = static_cast<int>(a[16]);
+    const double depth = ph.penetration_depth;
+    if (depth >= 0 && depth < a[15]) temporal(n, std::min(n - 1, static_cast<int>(depth / a[17])));
+    base += static_cast<int64_t>(Nt) * n;
+  }
+  if (a[4] != 0.0) {
+    const int n = static_cast<int>(a[19]);
+    const int idx = static_cast<int>(arrival / a[20]);
+    if (idx >= 1 && idx < n) g[base + idx] += 1.0;
+    base += n;
+  }
+  if (a[5] != 0.0) {
+    const int n = static_cast<int>(a[22]);
+    const double w = ph.weight;
+    if (w >= 0 && w < a[21]) g[base + std::min(n - 1, static_cast<int>(w / a[23]))] += 1.0;
+  }
+  hits += 1;
+}
+
 // SensorsGroup::record_hit (detector.cpp:178-270).
 bool record_hit(const Scene& S, Photon& ph, Accum& acc) {
   bool killed = false;
@@ -812,6 +872,9 @@ bool record_hit(const Scene& S, Photon& ph, Accum& acc) {
         case LMC_SENSOR_REFLECTANCE:
           g[0] += ph.weight;
           acc.hits[si] += 1;
+          break;
+        case LMC_SENSOR_STATISTICS:
+          stats_process_hit(S.sensors[si], ph, info, g, acc.hits[si]);
           break;
         default:
           break;
@@ -930,6 +993,7 @@ void run_photon(const Scene& S, Photon& ph, Xoshiro& rng, Accum& acc, bool track
     ph.pos.x += dx * step;
     ph.pos.y += dy * step;
     ph.pos.z += dz * step;
+    if (ph.pos.z > ph.penetration_depth) ph.penetration_depth = ph.pos.z;
     if (record_hit(S, ph, acc)) {
       ph.alive = false;
       break;
@@ -1113,6 +1177,7 @@ Scene build_scene(const lmc_scene_desc* d) {
   for (int i = 0; i < d->n_sensors; ++i) {
     Sensor s;
     s.d = d->sensors[i];
+    if (s.d.kind == LMC_SENSOR_STATISTICS) s.stat.assign(d->tables + s.d.stat_offset, d->tables + s.d.stat_offset + 27);
     if (s.d.kind == LMC_SENSOR_FARFIELD_CBS) {  // detector.cpp:884-907, 909-943
       const double dth = s.d.g[3], dph = s.d.g[4];
       for (int it = 0; it < s.d.n[0]; ++it) {

@@ -289,3 +289,47 @@ def test_estimator_agrees_with_analog_detection():
     # the reference's own cap: rows past int(theta_max / d_theta) stay empty
     out, _, _, _ = oracle.run(_estimator_scene(True), 50, 1)
     assert np.all(out.reshape(8, -1)[:, 11:] == 0.0)
+
+
+def _stats_scene():
+    """Plain Mie medium with a StatisticsSensor at z = 0 holding every histogram
+    (time-resolved) and the analog reflectance beside it."""
+    import luminis_mc as lmc
+    import scenes
+    from luminis_mc.simulation import SceneBuild
+    cfg, _ = scenes.config_plain_mie(1)
+    st = lmc.StatisticsSensor(0.0, absorb=True)
+    st.set_time_resolution(60.0, 15.0)
+    st.set_events_histogram_bins(1200)
+    st.set_theta_histogram_bins(0.0, math.pi / 2, 18)
+    st.set_phi_histogram_bins(0.0, 2 * math.pi, 12)
+    st.set_depth_histogram_bins(60.0, 25)
+    st.set_time_histogram_bins(100.0, 20)
+    st.set_weight_histogram_bins(1.0, 10)
+    g = lmc.SensorsGroup()
+    st = g.add_detector(st)
+    g.add_detector(lmc.ReflectanceSensor(0.0, absorb=False))
+    b = SceneBuild(cfg.sample, cfg.laser, g.detectors)
+    b._keep = cfg._keep
+    return b, st
+
+
+def test_statistics_sensor_layout_and_totals():
+    """StatisticsSensor (detector.cpp:1571-1888): every photon crossing the
+    plane backwards lands once in the time-integrated row of the events, theta,
+    phi and depth histograms (all ranges cover the exits) and of the weight
+    histogram; the time windows split the integrated row."""
+    b, st = _stats_scene()
+    out, hits, _, _ = oracle.run(b, 4000, 5)
+    off, n = b.offsets[0]
+    st.buffer[:] = out[off:off + n]
+    h = int(hits[0])
+    assert h == int(hits[1]) and h > 100
+    assert n == 5 * 1200 + 5 * 18 + 5 * 12 + 5 * 25 + 20 + 10
+    for name in ("events", "theta", "phi", "depth"):
+        rows = np.array(getattr(st, f"{name}_histogram"))
+        assert rows.shape[0] == 5
+        assert rows[0].sum() == h, name
+        assert rows[1:].sum() <= h and rows[1:].sum() > 0
+    assert sum(st.weight_histogram) == h
+    assert 0 < sum(st.time_histogram) <= h
