@@ -169,6 +169,20 @@ typedef struct lmc_laser_desc {
   double pulse_duration, repetition_rate, time_offset; /* ps, Hz, ps        */
 } lmc_laser_desc;
 
+/* Absorption r-z(-t) grid (Absorption, src/absortion.cpp:13-130): at every
+ * scattering vertex the absorbed weight w * mu_a / mu_t of the layer (1 when
+ * mu_t == 0, simulation.cpp:577-592) is added to bin (floor(r / d_r),
+ * floor(z / d_z)) of time slice 0 and, when d_t > 0, of slice
+ * floor(t / d_t) + 1.  Output n_t * n_r * n_z doubles, [t][r][z], at
+ * out_offset of the scene's accumulators; n_r = radius / d_r + 1, n_z =
+ * depth / d_z + 1, n_t = ceil(t_max / d_t) + 1 (1 when d_t == 0). */
+typedef struct lmc_absorption_desc {
+  int32_t enabled;
+  int32_t n_r, n_z, n_t;
+  double radius, depth, d_r, d_z, d_t, t_max;
+  int64_t out_offset;
+} lmc_absorption_desc;
+
 typedef struct lmc_scene_desc {
   int32_t n_layers, n_species, n_sensors, reserved0;
   const lmc_layer_desc* layers;
@@ -176,9 +190,10 @@ typedef struct lmc_scene_desc {
   const lmc_sensor_desc* sensors;
   const double* tables;
   int64_t n_tables;     /* doubles in `tables`                                */
-  int64_t n_out;        /* total accumulator doubles over all sensors         */
+  int64_t n_out;        /* total accumulator doubles (sensors, absorption grid) */
   lmc_laser_desc laser;
   double n_medium;      /* host index (Sample::refractive_index)              */
+  lmc_absorption_desc absorption; /* SimConfig::absorption; single-scene runs */
 } lmc_scene_desc;
 
 /* Precision / RNG modes. */

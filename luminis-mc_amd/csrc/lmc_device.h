@@ -48,6 +48,7 @@ template <> struct M<float> {
   static __device__ __forceinline__ void sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
   static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+  static __device__ __forceinline__ float floor(float x) { return floorf(x); }
 };
 template <> struct M<double> {
   static __device__ __forceinline__ double log(double x) { return ::log(x); }
@@ -62,6 +63,7 @@ template <> struct M<double> {
   static __device__ __forceinline__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
   static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double floor(double x) { return ::floor(x); }
 };
 
 template <class R> struct cx {

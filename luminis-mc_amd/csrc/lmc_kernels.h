@@ -61,6 +61,9 @@ template <class R> struct KParams {
   const LaunchConst<R>* lcs;     // one per scene
   const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
   const R* aux;                  // estimator bin geometry and statistics histogram parameters
+  R abs_radius, abs_depth, abs_dr, abs_dz, abs_dt;  // Absorption grid of a single-scene run
+  int32_t abs_on, abs_nr, abs_nz, abs_nt;
+  int64_t abs_off;
   int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
 };
 
@@ -1039,7 +1042,7 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
-template <class R, bool kXo, bool kTrack, bool kAdj, int kPart>
+template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, bool kAbs = false>
 __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
   // kPart 0: the whole event; 1: the flight up to the species of the next
   // vertex (left in ph.cur_sp); 2: the scatter at that vertex.  Estimator
@@ -1273,7 +1276,25 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   }
   // weight, roulette, cap (simulation.cpp:578-613)
   ph.events++;
+  const R w_in = ph.w;
   if (Ld.mu_t > R(0)) ph.w *= Ld.albedo;
+  if constexpr (kAbs) {  // Absorption::record_absorption (absortion.cpp:47-77) of w * mu_a / mu_t
+    const R d_w = Ld.mu_t > R(0) ? w_in * (Ld.mu_a / Ld.mu_t) : R(1);
+    const R r = M<R>::sqrt(ph.pos.x * ph.pos.x + ph.pos.y * ph.pos.y), z = ph.pos.z;
+    if (!(r > p.abs_radius) && z >= R(0) && !(z > p.abs_depth)) {
+      const int ir = static_cast<int>(r / p.abs_dr), iz = static_cast<int>(z / p.abs_dz);
+      if (ir < p.abs_nr && iz < p.abs_nz) {
+        double* g = p.out + p.abs_off;
+        const int64_t cell = static_cast<int64_t>(ir) * p.abs_nz + iz;
+        unsafeAtomicAdd(g + cell, static_cast<double>(d_w));
+        if (p.abs_dt > R(0)) {
+          const int k = static_cast<int>(M<R>::floor((ph.t0 + ph.gpl) / p.abs_dt)) + 1;
+          if (k >= 1 && k < p.abs_nt)
+            unsafeAtomicAdd(g + static_cast<int64_t>(k) * p.abs_nr * p.abs_nz + cell, static_cast<double>(d_w));
+        }
+      }
+    }
+  }
   if (ph.w < static_cast<R>(1e-4)) {
     R ur;
     if constexpr (kXo) {
@@ -1336,8 +1357,10 @@ template <class R> struct LdsHdr {
 // The body is shared by two entry points: 512-thread workgroups (two per CU
 // when their LDS allows) and 1024-thread workgroups (one per CU, the whole
 // LDS for tables that two workgroups could not hold).
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false>
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false,
+          bool kAbs = false>
 __device__ __forceinline__ void transport_body(const KParams<R>& p) {
+  static_assert(!kAbs || (!kBatch && !kAdj), "the Absorption grid is recorded by single-scene forward runs");
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1489,13 +1512,13 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     }
     int32_t ev0 = ph.events;
     if constexpr (kEst) {
-      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1>(p, v, ph);
+      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAbs>(p, v, ph);
       // SensorsGroup::run_estimators (detector.cpp:272-281) with the whole wave
       for (int si = 0; si < p.n_sensors; ++si)
         if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_wave<R, kXo>(p, v, ph, si);
-      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2>(p, v, ph);
+      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAbs>(p, v, ph);
     } else if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj, 0>(p, v, ph);
+      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAbs>(p, v, ph);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));
@@ -1525,26 +1548,28 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
 }
 
 // FP32 forward runs: at most 128 VGPRs, so 16 waves of a CU stay resident
-// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch>
+// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.  kAbs
+// instantiations record the Absorption grid at every vertex; runs without
+// one take the kernels that carry no trace of it.
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kAbs = false>
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (kAdj ? 3 : 4) : 2)))
 k_transport(KParams<R> p) {
-  transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch>(p);
+  transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch, false, kAbs>(p);
 }
 
-template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch>
+template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch, bool kAbs = false>
 __global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
-  transport_body<R, kXo, kTrack, true, kAdj, kBatch>(p);
+  transport_body<R, kXo, kTrack, true, kAdj, kBatch, false, kAbs>(p);
 }
 
 // Estimator runs (a FARFIELD_CBS sensor in estimator mode): the forced-scatter
 // loop over the sensor's bins dominates, so 256-thread workgroups and up to
 // 168 VGPRs (3 waves per SIMD); tables in LDS when they fit beside the grid.
-template <class R, bool kXo, bool kLdsTables>
+template <class R, bool kXo, bool kLdsTables, bool kAbs = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2)))
 k_transport_est(KParams<R> p) {
-  transport_body<R, kXo, true, kLdsTables, false, false, true>(p);
+  transport_body<R, kXo, true, kLdsTables, false, false, true, kAbs>(p);
 }
 
 // Sensor kernel: one thread per queued crossing.  For every sensor that

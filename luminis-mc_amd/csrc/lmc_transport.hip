@@ -123,6 +123,7 @@ struct SceneInfo {
   int n_single_planes = 0;     // planes a photon crosses at most once (records <= 1)
   int n_multi_planes = 0;      // interior non-absorbing planes (records <= max_events + 2)
   lmc_laser_desc laser{};
+  lmc_absorption_desc absorption{};  // out_offset relative to out_base
 };
 
 }  // namespace
@@ -431,6 +432,14 @@ int validate(const lmc_scene_desc* d) {
     if (s.estimator && s.kind == LMC_SENSOR_FARFIELD_CBS && s.z != 0.0)
       return fail(LMC_ERR_INVALID, "the CBS estimator is defined for a sensor at z = 0");
   }
+  const lmc_absorption_desc& A = d->absorption;
+  if (A.enabled) {
+    if (!(A.d_r > 0) || !(A.d_z > 0) || !(A.radius >= 0) || !(A.depth >= 0) || A.n_r < 1 || A.n_z < 1 || A.n_t < 1)
+      return fail(LMC_ERR_INVALID, "absorption grid needs positive bin sizes");
+    const int64_t n = static_cast<int64_t>(A.n_r) * A.n_z * A.n_t;
+    if (A.out_offset < 0 || A.out_offset + n > d->n_out)
+      return fail(LMC_ERR_INVALID, "absorption grid accumulators out of bounds");
+  }
   return LMC_OK;
 }
 
@@ -495,6 +504,48 @@ int64_t records_per_photon(const SceneInfo& sc, int max_events) {
 // scene i, photon indices photon_offset + [0, budgets[i]) within each scene, in
 // launches over the concatenated index space; use_graph captures the launch
 // sequence once into a hipGraph and replays it while (budgets, run) repeat.
+template <class R> using TransportFn = void (*)(KParams<R>);
+
+// The transport kernel of a launch: the estimator, 1024-thread or
+// 512/256-thread body; a batch or a single scene; with or without reverse-path
+// tracking; tables in LDS or read through the caches.  kAbs: single-scene
+// forward runs recording the Absorption grid (nullptr where that does not apply).
+template <class R, bool kXo, bool kAdj, bool kAbs>
+TransportFn<R> pick_transport(bool est, bool wide, bool lds_tables, bool batch, bool track) {
+  if constexpr (kAbs && kAdj) {
+    return nullptr;
+  } else {
+    if (est) {
+      if constexpr (kAdj) return nullptr;
+      else return lds_tables ? k_transport_est<R, kXo, true, kAbs> : k_transport_est<R, kXo, false, kAbs>;
+    }
+    if (batch) {
+      if constexpr (kAbs) {
+        return nullptr;
+      } else {
+        if (wide) {
+          if constexpr (sizeof(R) == 4 && !kAdj)
+            return track ? k_transport_wide<R, kXo, true, kAdj, true> : k_transport_wide<R, kXo, false, kAdj, true>;
+          else return nullptr;
+        }
+        if (track)
+          return lds_tables ? k_transport<R, kXo, true, true, kAdj, true> : k_transport<R, kXo, true, false, kAdj, true>;
+        return lds_tables ? k_transport<R, kXo, false, true, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
+      }
+    }
+    if (wide) {
+      if constexpr (sizeof(R) == 4 && !kAdj)
+        return track ? k_transport_wide<R, kXo, true, kAdj, false, kAbs> : k_transport_wide<R, kXo, false, kAdj, false, kAbs>;
+      else return nullptr;
+    }
+    if (track)
+      return lds_tables ? k_transport<R, kXo, true, true, kAdj, false, kAbs>
+                        : k_transport<R, kXo, true, false, kAdj, false, kAbs>;
+    return lds_tables ? k_transport<R, kXo, false, true, kAdj, false, kAbs>
+                      : k_transport<R, kXo, false, false, kAdj, false, kAbs>;
+  }
+}
+
 template <class R, bool kXo, bool kAdj>
 int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph,
              lmc_stats* st) {
@@ -512,6 +563,19 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.planes = s->planes_dev.p;
   p.lcs = T.lcs.p;
   p.aux = T.aux.p;  // estimator bin geometry, statistics histogram parameters
+  if (!s->scenes.empty() && s->scenes[0].absorption.enabled && s->scenes.size() == 1) {
+    const lmc_absorption_desc& A = s->scenes[0].absorption;
+    p.abs_on = 1;
+    p.abs_nr = A.n_r;
+    p.abs_nz = A.n_z;
+    p.abs_nt = A.n_t;
+    p.abs_radius = static_cast<R>(A.radius);
+    p.abs_depth = static_cast<R>(A.depth);
+    p.abs_dr = static_cast<R>(A.d_r);
+    p.abs_dz = static_cast<R>(A.d_z);
+    p.abs_dt = static_cast<R>(A.d_t);
+    p.abs_off = s->scenes[0].out_base + A.out_offset;
+  }
   p.n_layers = static_cast<int32_t>(s->layers.size());
   p.n_species = static_cast<int32_t>(s->species.size());
   p.n_sensors = static_cast<int32_t>(s->sensors_host.size());
@@ -582,6 +646,13 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     return fail(LMC_ERR_UNSUPPORTED, "the CBS estimator runs single scenes without gradients");
   if (est && !r->track_reverse_paths)
     return fail(LMC_ERR_INVALID, "the CBS estimator needs track_reverse_paths (reverse path of every forced scatter)");
+  const bool abs_run = p.abs_on != 0;
+  if (abs_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by forward runs");
+  const bool track = r->track_reverse_paths != 0;
+  auto pick = [&](bool wide, bool lds_t) {
+    return abs_run ? pick_transport<R, kXo, kAdj, true>(est, wide, lds_t, batch, track)
+                   : pick_transport<R, kXo, kAdj, false>(est, wide, lds_t, batch, track);
+  };
   const bool fit512 = 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
   const bool wide_ok = !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
   int block = r->block_size > 0 ? r->block_size : (fit512 ? 512 : (wide_ok ? 1024 : 256));
@@ -602,7 +673,8 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     const size_t est_need = base_lds + al16(sizeof(double) * s->est_lds_n);
     lds_tables = 3 * (est_need + tab_lds) <= g_max_lds_cu && est_need + tab_lds <= g_max_lds_block;
     lds = lds_tables ? est_need + tab_lds : est_need;
-    if constexpr (!kAdj) kt_est = lds_tables ? k_transport_est<R, kXo, true> : k_transport_est<R, kXo, false>;
+    kt_est = pick(false, lds_tables);
+    if (!kt_est) return fail(LMC_ERR_UNSUPPORTED, "no estimator kernel for this run");
     int rcx = raise_lds_limit(kt_est, lds);
     if (rcx) return rcx;
     int per_cu = 0;
@@ -612,7 +684,6 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   }
   const int grid = g_num_cu * bpc;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
-  const bool track = r->track_reverse_paths != 0;
   int64_t chunk = static_cast<int64_t>(total);
   p.q_chunk = kWaveQueueChunk;
   // slots a launch can leave unused: the rest of every wave's last chunk
@@ -630,25 +701,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     p.q_cap = s->q_cap;
   }
   p.q_count = s->qcount.p;
-  void (*kt)(KParams<R>) = nullptr;
-  if (est) {
-    kt = kt_est;
-  } else if (block > 512) {
-    if constexpr (sizeof(R) == 4 && !kAdj) {
-      if (batch)
-        kt = track ? k_transport_wide<R, kXo, true, kAdj, true> : k_transport_wide<R, kXo, false, kAdj, true>;
-      else
-        kt = track ? k_transport_wide<R, kXo, true, kAdj, false> : k_transport_wide<R, kXo, false, kAdj, false>;
-    }
-  } else if (batch) {
-    if (track)
-      kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, true> : k_transport<R, kXo, true, false, kAdj, true>;
-    else
-      kt = lds_tables ? k_transport<R, kXo, false, true, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
-  } else if (track)
-    kt = lds_tables ? k_transport<R, kXo, true, true, kAdj, false> : k_transport<R, kXo, true, false, kAdj, false>;
-  else
-    kt = lds_tables ? k_transport<R, kXo, false, true, kAdj, false> : k_transport<R, kXo, false, false, kAdj, false>;
+  const TransportFn<R> kt = est ? kt_est : pick(block > 512, lds_tables);
   if (!kt) return fail(LMC_ERR_UNSUPPORTED, "no transport kernel for this launch shape");
   int rc = raise_lds_limit(kt, lds);
   if (rc) return rc;
@@ -873,6 +926,9 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
   sc.launch_layer = sc.layer_base + ll;
   sc.out_base = s->n_out;
   sc.n_out = d->n_out;
+  if (d->absorption.enabled && !privatise)
+    return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by single-scene runs only");
+  sc.absorption = d->absorption;
   sc.n_sensors = d->n_sensors;
   sc.laser = d->laser;
   const int64_t tbase = static_cast<int64_t>(s->tables.size());

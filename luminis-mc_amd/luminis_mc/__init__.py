@@ -23,6 +23,7 @@ from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFi
                       solid_angle_bin)
 from .simulation import (GpuBatch, GpuScene, SimConfig, device_info, run_simulation_batched, run_simulation_parallel,
                          validate_sim_config)
+from .absorption import Absorption
 from .tuning import autotune
 from .adjoint import DifferentiableTransport, crn_central_difference
 

@@ -85,6 +85,14 @@ class LaserDesc(ctypes.Structure):
     ]
 
 
+class AbsorptionDesc(ctypes.Structure):
+    _fields_ = [
+        ("enabled", c_int32), ("n_r", c_int32), ("n_z", c_int32), ("n_t", c_int32),
+        ("radius", c_double), ("depth", c_double), ("d_r", c_double), ("d_z", c_double),
+        ("d_t", c_double), ("t_max", c_double), ("out_offset", c_int64),
+    ]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("n_layers", c_int32), ("n_species", c_int32), ("n_sensors", c_int32), ("reserved0", c_int32),
@@ -92,6 +100,7 @@ class SceneDesc(ctypes.Structure):
         ("sensors", ctypes.POINTER(SensorDesc)), ("tables", c_dp),
         ("n_tables", c_int64), ("n_out", c_int64),
         ("laser", LaserDesc), ("n_medium", c_double),
+        ("absorption", AbsorptionDesc),
     ]
 
 

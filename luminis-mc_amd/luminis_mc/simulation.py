@@ -59,7 +59,7 @@ def validate_sim_config(config: SimConfig) -> None:
 class SceneBuild:
     """Flat descriptor of one scene plus the arrays it points into."""
 
-    def __init__(self, sample: Sample, laser: Laser, sensors: list[Sensor]):
+    def __init__(self, sample: Sample, laser: Laser, sensors: list[Sensor], absorption=None):
         if not sample.layers:
             raise ValueError("Sample has no layers")
         tables: list[np.ndarray] = []
@@ -154,6 +154,10 @@ class SceneBuild:
             self.offsets.append((n_out, s.out_len()))
             n_out += s.out_len()
 
+        self.absorption = absorption
+        self.abs_offset = n_out
+        if absorption is not None:
+            n_out += absorption.buffer.size
         self.tables = np.concatenate(tables) if tables else np.zeros(1)
         self.species = sp_arr
         self.layers = ly_arr
@@ -193,6 +197,8 @@ class SceneBuild:
         ld.repetition_rate = laser.repetition_rate
         ld.time_offset = laser.time_offset
         desc.n_medium = sample.refractive_index
+        if absorption is not None:
+            absorption._fill_desc(desc.absorption, self.abs_offset)
         self.desc = desc
 
     def scatter_results(self, out: np.ndarray, hits: np.ndarray) -> None:
@@ -200,6 +206,8 @@ class SceneBuild:
         for s, (off, n), h in zip(self.sensors, self.offsets, hits):
             s.buffer += out[off:off + n]
             s.hits += int(h)
+        if self.absorption is not None:
+            self.absorption.buffer += out[self.abs_offset:self.abs_offset + self.absorption.buffer.size]
 
 
 def run_desc_from(config: SimConfig, n_photons: int | None = None, photon_offset: int | None = None):
@@ -226,8 +234,8 @@ def run_desc_from(config: SimConfig, n_photons: int | None = None, photon_offset
 class GpuScene:
     """A scene resident on the device: create once, run many times."""
 
-    def __init__(self, sample: Sample, laser: Laser, sensors, build: SceneBuild | None = None):
-        self.build = build if build is not None else SceneBuild(sample, laser, list(sensors))
+    def __init__(self, sample: Sample, laser: Laser, sensors, build: SceneBuild | None = None, absorption=None):
+        self.build = build if build is not None else SceneBuild(sample, laser, list(sensors), absorption)
         self._h = ctypes.c_void_p()
         _lib.check(_lib.lib().lmc_scene_create(ctypes.byref(self.build.desc), ctypes.byref(self._h)),
                    "lmc_scene_create")
@@ -345,7 +353,7 @@ def run_simulation_batched(configs, use_graph: bool = True) -> dict:
     for c in configs:
         validate_sim_config(c)
         if c.absorption is not None:
-            raise NotImplementedError("the Absorption grid is not run by the GPU transport path")
+            raise NotImplementedError("the Absorption grid is recorded by single-scene runs only")
     keys = ("seed", "MAX_EVENTS", "track_reverse_paths", "precision", "rng", "photon_offset",
             "block_size", "blocks_per_cu", "claim")
     c0 = configs[0]
@@ -368,9 +376,7 @@ def run_simulation_parallel(config: SimConfig) -> None:
     """Transport config.n_photons photons on the GPU and add the results into
     config.detector's sensors (python_module.cpp:664-673 -> simulation.cpp:105-249)."""
     validate_sim_config(config)
-    if config.absorption is not None:
-        raise NotImplementedError("the Absorption grid is not run by the GPU transport path")
-    scene = GpuScene(config.sample, config.laser, config.detector.detectors)
+    scene = GpuScene(config.sample, config.laser, config.detector.detectors, absorption=config.absorption)
     try:
         config.last_stats = scene.run(run_desc_from(config))
         scene.merge_into_sensors()

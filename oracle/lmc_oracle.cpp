@@ -20,6 +20,7 @@
 //                                (process_estimation and its helpers)
 //   src/detector.cpp:1703-1800   StatisticsSensor::process_hit (and the
 //                                penetration depth, simulation.cpp:396-400)
+//   src/absortion.cpp:47-77      Absorption::record_absorption
 //   src/sample.cpp:90-104,221-288 layer navigation, mixture species draw
 //   src/medium.cpp:123-126,175-191 free path, S(theta) lookup
 //   src/phase.cpp:27-49,108-115  azimuth rejection sampler, HG inversion
@@ -267,7 +268,9 @@ struct Scene {
   lmc_laser_desc laser;
   double light_speed{1.0};
   int64_t n_out{0};
+  lmc_absorption_desc absorption{};
 };
+
 
 size_t layer_index_at(const Scene& S, double z) {  // sample.cpp:221-235
   const size_t L = S.layers.size();
@@ -333,6 +336,23 @@ struct Photon {
   C2 Traw, Traw_buf;
   bool has_T_prev{false};
 };
+
+// Absorption::record_absorption (src/absortion.cpp:47-77).
+void record_absorption(const Scene& S, const Photon& ph, double d_weight, double* out) {
+  const lmc_absorption_desc& A = S.absorption;
+  const double r = std::sqrt(ph.pos.x * ph.pos.x + ph.pos.y * ph.pos.y), z = ph.pos.z;
+  if (r > A.radius || z < 0.0 || z > A.depth) return;
+  const size_t i_r = static_cast<size_t>(std::floor(r / A.d_r));
+  const size_t i_z = static_cast<size_t>(std::floor(z / A.d_z));
+  if (i_r >= static_cast<size_t>(A.n_r) || i_z >= static_cast<size_t>(A.n_z)) return;
+  double* g = out + A.out_offset;
+  g[i_r * A.n_z + i_z] += d_weight;
+  if (A.d_t > 0.0) {
+    const double time = ph.launch_time + (ph.opticalpath / ph.velocity);
+    const int k = static_cast<int>(std::floor(time / A.d_t)) + 1;
+    if (k >= 1 && k < A.n_t) g[(static_cast<size_t>(k) * A.n_r + i_r) * A.n_z + i_z] += d_weight;
+  }
+}
 
 // Per-thread accumulators (the clone of the SensorsGroup, detector.cpp:299-307).
 struct Accum {
@@ -1117,10 +1137,14 @@ void run_photon(const Scene& S, Photon& ph, Xoshiro& rng, Accum& acc, bool track
     ph.events++;
     acc.events++;
     const double mu_t = sl.d.mu_t;
+    double d_weight = 1.0;  // simulation.cpp:577-592
     if (mu_t > 0) {
+      const double absorption_prob = sl.d.mu_a / mu_t;
       const double scat = sl.d.mu_s / mu_t;
+      d_weight = ph.weight * absorption_prob;
       ph.weight *= scat;
     }
+    if (S.absorption.enabled) record_absorption(S, ph, d_weight, acc.out.data());
     if (ph.weight < 1e-4) {
       if (rng.uniform() < 0.1) {
         ph.weight /= 0.1;
@@ -1221,6 +1245,7 @@ Scene build_scene(const lmc_scene_desc* d) {
   S.laser = d->laser;
   S.light_speed = 1.0 / d->n_medium;
   S.n_out = d->n_out;
+  S.absorption = d->absorption;
   return S;
 }
 

@@ -11,7 +11,6 @@ the FP64 transport on the same per-photon random streams: a relative step of 1e-
 perturbed coefficients would bias the difference quotient by several percent."""
 import numpy as np
 import pytest
-import torch
 
 import luminis_mc as lmc
 import scenes
@@ -65,6 +64,9 @@ def test_adjoint_matches_crn_finite_differences(require_gpu):
 
 
 def test_autograd_backward_applies_the_jacobian(require_gpu):
+    # torch on the CPU only, imported after the library has taken the device:
+    # importing it first would bind the library to torch's bundled HIP runtime
+    torch = pytest.importorskip("torch")
     sample, laser, sensors, keep = scenes.scene_adjoint()
     tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=500_000, seed=7)
     th0 = tr.theta0()
