@@ -103,7 +103,7 @@ template <class R> __device__ __forceinline__ Quad<R> get4(const R* f, int q) {
     return {u.x, u.y, v.x, v.y};
   }
 }
-enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, I_DEPTH = 4, kRecI = 5 };  // I_DEPTH: float bits
+enum RecI { I_PLANE = 0, I_MASK = 1, I_EVENTS = 2, I_SP = 3, I_DEPTH = 4, kRecI = 5 };  // I_DEPTH: float bits (kernels with kAux >= 1)
 
 template <class R> struct SceneView {
   const LaunchConst<R>* lc;
@@ -598,7 +598,7 @@ __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R ar
 // overshoot-corrected paths (detector.cpp:194-235); the sensors' filters
 // decide acceptance and absorption here, the deposits happen in
 // k_crossings.  Returns true when an absorbing sensor accepted the photon.
-template <class R, bool kXo, bool kAdj>
+template <class R, bool kXo, bool kAdj, int kAux>
 __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
                                           V3<R> prev, int cdir) {
   const R zp = static_cast<R>(v.planes[pl].z);
@@ -688,13 +688,13 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   iq[I_MASK * c] = mask;
   iq[I_EVENTS * c] = ph.events;
   iq[I_SP * c] = ph.sp;
-  iq[I_DEPTH * c] = __float_as_int(static_cast<float>(ph.depth));
+  if constexpr (kAux >= 1) iq[I_DEPTH * c] = __float_as_int(static_cast<float>(ph.depth));
   return killed;
 }
 
 // SensorsGroup::record_hit (detector.cpp:178-270) for the segment prev->pos:
 // planes within [min z, max z] in increasing z.
-template <class R, bool kXo, bool kAdj>
+template <class R, bool kXo, bool kAdj, int kAux>
 __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, V3<R> prev) {
   const R z1 = prev.z, z2 = ph.pos.z;
   if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
@@ -704,7 +704,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   for (int pl = pl0; pl < pl1; ++pl) {
     const R zp = static_cast<R>(v.planes[pl].z);
     if (zp >= zmin && zp <= zmax)
-      killed |= plane_hit<R, kXo, kAdj>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
+      killed |= plane_hit<R, kXo, kAdj, kAux>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
   }
   return killed;
 }
@@ -1042,7 +1042,7 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
-template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, bool kAbs = false>
+template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, int kAux = 1>
 __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
   // kPart 0: the whole event; 1: the flight up to the species of the next
   // vertex (left in ph.cur_sp); 2: the scatter at that vertex.  Estimator
@@ -1102,7 +1102,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
         ph.pos.z = zi + (up ? 1e-12 : -1e-12);
       else
         ph.pos.z = zi;
-      if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev)) {
         ph.alive = false;
         return 0;
       }
@@ -1119,8 +1119,10 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       ph.pos.x += dir.x * step;
       ph.pos.y += dir.y * step;
       ph.pos.z += dir.z * step;
-      if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
-      if (record_hit<R, kXo, kAdj>(p, v, ph, prev)) {
+      if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
+        if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
+      }
+      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev)) {
         ph.alive = false;
         return 0;
       }
@@ -1278,7 +1280,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   ph.events++;
   const R w_in = ph.w;
   if (Ld.mu_t > R(0)) ph.w *= Ld.albedo;
-  if constexpr (kAbs) {  // Absorption::record_absorption (absortion.cpp:47-77) of w * mu_a / mu_t
+  if constexpr (kAux == 2) {  // Absorption::record_absorption (absortion.cpp:47-77) of w * mu_a / mu_t
     const R d_w = Ld.mu_t > R(0) ? w_in * (Ld.mu_a / Ld.mu_t) : R(1);
     const R r = M<R>::sqrt(ph.pos.x * ph.pos.x + ph.pos.y * ph.pos.y), z = ph.pos.z;
     if (!(r > p.abs_radius) && z >= R(0) && !(z > p.abs_depth)) {
@@ -1357,10 +1359,9 @@ template <class R> struct LdsHdr {
 // The body is shared by two entry points: 512-thread workgroups (two per CU
 // when their LDS allows) and 1024-thread workgroups (one per CU, the whole
 // LDS for tables that two workgroups could not hold).
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false,
-          bool kAbs = false>
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false, int kAux = 1>
 __device__ __forceinline__ void transport_body(const KParams<R>& p) {
-  static_assert(!kAbs || (!kBatch && !kAdj), "the Absorption grid is recorded by single-scene forward runs");
+  static_assert(kAux != 2 || (!kBatch && !kAdj), "the Absorption grid is recorded by single-scene forward runs");
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1512,13 +1513,13 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     }
     int32_t ev0 = ph.events;
     if constexpr (kEst) {
-      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAbs>(p, v, ph);
+      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAux>(p, v, ph);
       // SensorsGroup::run_estimators (detector.cpp:272-281) with the whole wave
       for (int si = 0; si < p.n_sensors; ++si)
         if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_wave<R, kXo>(p, v, ph, si);
-      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAbs>(p, v, ph);
+      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAux>(p, v, ph);
     } else if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAbs>(p, v, ph);
+      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAux>(p, v, ph);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));
@@ -1548,28 +1549,30 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
 }
 
 // FP32 forward runs: at most 128 VGPRs, so 16 waves of a CU stay resident
-// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.  kAbs
-// instantiations record the Absorption grid at every vertex; runs without
-// one take the kernels that carry no trace of it.
-template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kAbs = false>
+// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.
+// kAux: per-photon outputs beyond the sensors' crossings.  1 tracks the
+// penetration depth (maximum z) that StatisticsSensor histograms; 2 also
+// records the Absorption grid at every vertex; 0 (FP32 Philox forward runs
+// without either) carries no trace of them.
+template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, int kAux = 1>
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (kAdj ? 3 : 4) : 2)))
 k_transport(KParams<R> p) {
-  transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch, false, kAbs>(p);
+  transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch, false, kAux>(p);
 }
 
-template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch, bool kAbs = false>
+template <class R, bool kXo, bool kTrack, bool kAdj, bool kBatch, int kAux = 1>
 __global__ void __launch_bounds__(1024) k_transport_wide(KParams<R> p) {
-  transport_body<R, kXo, kTrack, true, kAdj, kBatch, false, kAbs>(p);
+  transport_body<R, kXo, kTrack, true, kAdj, kBatch, false, kAux>(p);
 }
 
 // Estimator runs (a FARFIELD_CBS sensor in estimator mode): the forced-scatter
 // loop over the sensor's bins dominates, so 256-thread workgroups and up to
 // 168 VGPRs (3 waves per SIMD); tables in LDS when they fit beside the grid.
-template <class R, bool kXo, bool kLdsTables, bool kAbs = false>
+template <class R, bool kXo, bool kLdsTables, int kAux = 1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 2)))
 k_transport_est(KParams<R> p) {
-  transport_body<R, kXo, true, kLdsTables, false, false, true, kAbs>(p);
+  transport_body<R, kXo, true, kLdsTables, false, false, true, kAux>(p);
 }
 
 // Sensor kernel: one thread per queued crossing.  For every sensor that
@@ -1723,6 +1726,8 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   }
 }
 
+// Plain kernels: defined once, in the host translation unit.
+#ifndef LMC_TRANSPORT_PARTS
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -1759,5 +1764,7 @@ __global__ void k_philox_probe(int n, const uint4* ctr, uint32_t k0, uint32_t k1
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
 }
+
+#endif  // LMC_TRANSPORT_PARTS
 
 }  // namespace lmc

@@ -43,7 +43,7 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
-#include "lmc_kernels.h"
+#include "lmc_launch.h"
 
 using namespace lmc;
 
@@ -504,48 +504,6 @@ int64_t records_per_photon(const SceneInfo& sc, int max_events) {
 // scene i, photon indices photon_offset + [0, budgets[i]) within each scene, in
 // launches over the concatenated index space; use_graph captures the launch
 // sequence once into a hipGraph and replays it while (budgets, run) repeat.
-template <class R> using TransportFn = void (*)(KParams<R>);
-
-// The transport kernel of a launch: the estimator, 1024-thread or
-// 512/256-thread body; a batch or a single scene; with or without reverse-path
-// tracking; tables in LDS or read through the caches.  kAbs: single-scene
-// forward runs recording the Absorption grid (nullptr where that does not apply).
-template <class R, bool kXo, bool kAdj, bool kAbs>
-TransportFn<R> pick_transport(bool est, bool wide, bool lds_tables, bool batch, bool track) {
-  if constexpr (kAbs && kAdj) {
-    return nullptr;
-  } else {
-    if (est) {
-      if constexpr (kAdj) return nullptr;
-      else return lds_tables ? k_transport_est<R, kXo, true, kAbs> : k_transport_est<R, kXo, false, kAbs>;
-    }
-    if (batch) {
-      if constexpr (kAbs) {
-        return nullptr;
-      } else {
-        if (wide) {
-          if constexpr (sizeof(R) == 4 && !kAdj)
-            return track ? k_transport_wide<R, kXo, true, kAdj, true> : k_transport_wide<R, kXo, false, kAdj, true>;
-          else return nullptr;
-        }
-        if (track)
-          return lds_tables ? k_transport<R, kXo, true, true, kAdj, true> : k_transport<R, kXo, true, false, kAdj, true>;
-        return lds_tables ? k_transport<R, kXo, false, true, kAdj, true> : k_transport<R, kXo, false, false, kAdj, true>;
-      }
-    }
-    if (wide) {
-      if constexpr (sizeof(R) == 4 && !kAdj)
-        return track ? k_transport_wide<R, kXo, true, kAdj, false, kAbs> : k_transport_wide<R, kXo, false, kAdj, false, kAbs>;
-      else return nullptr;
-    }
-    if (track)
-      return lds_tables ? k_transport<R, kXo, true, true, kAdj, false, kAbs>
-                        : k_transport<R, kXo, true, false, kAdj, false, kAbs>;
-    return lds_tables ? k_transport<R, kXo, false, true, kAdj, false, kAbs>
-                      : k_transport<R, kXo, false, false, kAdj, false, kAbs>;
-  }
-}
-
 template <class R, bool kXo, bool kAdj>
 int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph,
              lmc_stats* st) {
@@ -649,9 +607,12 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   const bool abs_run = p.abs_on != 0;
   if (abs_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by forward runs");
   const bool track = r->track_reverse_paths != 0;
-  auto pick = [&](bool wide, bool lds_t) {
-    return abs_run ? pick_transport<R, kXo, kAdj, true>(est, wide, lds_t, batch, track)
-                   : pick_transport<R, kXo, kAdj, false>(est, wide, lds_t, batch, track);
+  bool has_stats = false;
+  for (const SensorDev& sd : s->sensors_host) has_stats = has_stats || sd.kind == LMC_SENSOR_STATISTICS;
+  // kernels without the penetration depth only where nothing reads it
+  const int aux_mode = abs_run ? 2 : ((has_stats || est || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
+  auto pick = [&](bool wide, bool lds_t) -> TransportFn<R> {
+    return transport_kernel<R, kXo, kAdj>(aux_mode, est, wide, lds_t, batch, track);
   };
   const bool fit512 = 2 * need <= g_max_lds_cu && need <= g_max_lds_block;
   const bool wide_ok = !kAdj && sizeof(R) == 4 && need <= g_max_lds_block && need <= g_max_lds_cu;
