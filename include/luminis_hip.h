@@ -95,8 +95,24 @@ enum {
                                       (no reference counterpart)             */
   LMC_SENSOR_REFLECTANCE = 5,    /* extension: sum of exit weights (the
                                     PhotonRecordSensor weight sum)          */
-  LMC_SENSOR_STATISTICS = 6      /* StatisticsSensor (detector.cpp:1571-1888) */
+  LMC_SENSOR_STATISTICS = 6,     /* StatisticsSensor (detector.cpp:1571-1888) */
+  LMC_SENSOR_PHOTON_RECORD = 7   /* PhotonRecordSensor (detector.cpp:427-485):
+                                    one record per accepted crossing, read with
+                                    lmc_scene_records; no accumulators      */
 };
+
+/* One PhotonRecordSensor record (PhotonRecord, photon.hpp:259-291), doubles:
+ *  [0] index of the sensor in its scene's sensor list   [1] events
+ *  [2] penetration_depth  [3] launch_time  [4] arrival_time (launch_time +
+ *  opticalpath, velocity 1)  [5] opticalpath (geometric path at the hit)
+ *  [6] weight  [7] k = 2 pi / wavelength
+ *  [8..10] position_first_scattering  [11..13] position_last_scattering
+ *  [14..16] position_detector  [17..19] direction  [20..22] m  [23..25] n
+ *  [26..29] polarization_forward (m re, m im, n re, n im)
+ *  [30..33] polarization_reverse: the reverse-path Jones vector of this hit
+ *           when a FarFieldCBSSensor earlier in the plane's sensor list
+ *           binned it with events >= 2 (detector.cpp:1281-1288), else 0. */
+#define LMC_RECORD_FIELDS 34
 
 /* Crossing-direction filter values (detector.hpp CrossingDirection). */
 enum { LMC_DIR_FORWARD = 0, LMC_DIR_BACKWARD = 1, LMC_DIR_BOTH = 2 };
@@ -275,8 +291,15 @@ int lmc_scene_run(lmc_scene* s, const lmc_run_desc* run, lmc_stats* stats);
  * (n_sensors entries); either may be NULL.
  * Replaces: SensorsGroup::merge_from (detector.cpp:283-297). */
 int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits);
-/* Zero the device accumulators and hit counters. */
+/* Zero the device accumulators and hit counters (and drop the records). */
 int lmc_scene_zero(lmc_scene* s);
+/* PhotonRecordSensor records gathered since creation or the last zero, in
+ * crossing-queue order (not photon order): copies min(max_records, count)
+ * records of LMC_RECORD_FIELDS doubles into `out` (may be NULL to count
+ * only) and sets *n_records to the count.  Single-scene forward runs only.
+ * Replaces: PhotonRecordSensor::recorded_photons (detector.hpp:280-298) and
+ * its merge_from (detector.cpp:455-461). */
+int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n_records);
 
 /* ---- batched persistent launch over many independent scenes ----------- */
 /* No counterpart in the reference (which runs one SimConfig per call): up to

@@ -332,9 +332,13 @@ struct SensorDev {
   float q_cmin, q_x0, q_x1, q_y0, q_y1, q_t0, q_t1, pad3;
 };
 
+// flags: kPlaneCbs a FarFieldCBSSensor (analog) reads the reverse-path
+// history of its crossings; kPlaneRec a PhotonRecordSensor reads r_1, r_n and
+// the launch time.
+enum : int32_t { kPlaneCbs = 1, kPlaneRec = 2 };
 struct PlaneDev {
   double z;
-  int32_t first, count, has_cbs, scene;  // scene: index of its laser constants
+  int32_t first, count, flags, scene;  // scene: index of its laser constants
 };
 
 }  // namespace lmc

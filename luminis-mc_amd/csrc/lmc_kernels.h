@@ -65,6 +65,11 @@ template <class R> struct KParams {
   int32_t abs_on, abs_nr, abs_nz, abs_nt;
   int64_t abs_off;
   int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
+  // PhotonRecordSensor records of one sensor-kernel launch (LMC_RECORD_FIELDS
+  // doubles each), [0] records written, [1] records that found no room
+  double* rec;
+  unsigned long long* rec_count;
+  int64_t rec_cap;
 };
 
 // One queued plane crossing accepted by at least one sensor of its plane:
@@ -670,12 +675,18 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   put4(f, 3, ph.P.n.y, ph.P.n.z, ph.P.s.x, ph.P.s.y);
   put4(f, 4, ph.P.s.z, ph.m1.x, ph.m1.y, ph.m1.z);
   p.qd[slot] = opl_hit;
-  if (ph.events >= 2 && v.planes[pl].has_cbs) {
+  if (ph.events >= 2 && (v.planes[pl].flags & kPlaneCbs)) {
     put4(f, 5, ph.s1.x, ph.s1.y, ph.s1.z, ph.mq.x);
     put4(f, 6, ph.mq.y, ph.mq.z, ph.sq.x, ph.sq.y);
     put4(f, 7, ph.sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r);
     put4(f, 8, ph.T[1].i, ph.T[2].r, ph.T[2].i, ph.T[3].r);
     put4(f, 9, ph.T[3].i, ph.rn.x - ph.r1.x, ph.rn.y - ph.r1.y, ph.rn.z - ph.r1.z);
+  }
+  if constexpr (kAux >= 1) {  // what a PhotonRecordSensor reads beyond the sensors' common record
+    if (v.planes[pl].flags & kPlaneRec) {
+      put4(f, 10, ph.r1.x, ph.r1.y, ph.r1.z, ph.t0);
+      put4(f, 11, ph.rn.x, ph.rn.y, ph.rn.z, gpl_hit);
+    }
   }
   if constexpr (kAdj) {
     R score[3 * kMaxGradLayers];
@@ -1631,10 +1642,39 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
     const cx<R> Efx = cadd(cscale(fm, P.m.x), cscale(fn, P.n.x));
     const cx<R> Efy = cadd(cscale(fm, P.m.y), cscale(fn, P.n.y));
     const int first = p.planes[pl].first, cnt = p.planes[pl].count;
+    // the photon's polarization_reverse as a PhotonRecordSensor sees it: set
+    // by a CBS sensor earlier in the plane's list (detector.cpp:1281-1288)
+    cx<R> pr_m{R(0), R(0)}, pr_n{R(0), R(0)};
     for (int k = 0; k < cnt; ++k) {
       if (!((mask >> k) & 1)) continue;
       const int si = first + k;
       const SensorDev& sd = p.sensors[si];
+      if (sd.kind == LMC_SENSOR_PHOTON_RECORD) {  // PhotonRecordSensor::process_hit (detector.cpp:463-485)
+        const unsigned long long slot = atomicAdd(&p.rec_count[0], 1ull);
+        if (static_cast<int64_t>(slot) < p.rec_cap) {
+          const Quad<R> q10 = get4(f, 10), q11 = get4(f, 11);
+          double* o = p.rec + static_cast<int64_t>(slot) * LMC_RECORD_FIELDS;
+          const double vals[LMC_RECORD_FIELDS] = {
+              static_cast<double>(sd.id), static_cast<double>(events),
+              static_cast<double>(__int_as_float(iq[I_DEPTH * c])), static_cast<double>(q10.d),
+              static_cast<double>(arr), static_cast<double>(q11.d), static_cast<double>(w), 0.0,
+              static_cast<double>(q10.a), static_cast<double>(q10.b), static_cast<double>(q10.c),
+              static_cast<double>(q11.a), static_cast<double>(q11.b), static_cast<double>(q11.c),
+              static_cast<double>(hx), static_cast<double>(hy), p.planes[pl].z,
+              static_cast<double>(P.s.x), static_cast<double>(P.s.y), static_cast<double>(P.s.z),
+              static_cast<double>(P.m.x), static_cast<double>(P.m.y), static_cast<double>(P.m.z),
+              static_cast<double>(P.n.x), static_cast<double>(P.n.y), static_cast<double>(P.n.z),
+              static_cast<double>(Em.r), static_cast<double>(Em.i), static_cast<double>(En.r),
+              static_cast<double>(En.i), static_cast<double>(pr_m.r), static_cast<double>(pr_m.i),
+              static_cast<double>(pr_n.r), static_cast<double>(pr_n.i)};
+#pragma unroll
+          for (int j = 0; j < LMC_RECORD_FIELDS; ++j) o[j] = vals[j];
+        } else {
+          atomicAdd(&p.rec_count[1], 1ull);
+        }
+        atomicAdd(&shits[si], 1u);
+        continue;
+      }
       if (sd.kind == LMC_SENSOR_STATISTICS) {
         stats_deposit<R, kAdj>(p, sd, sgrid, events, P.s, arr, w, __int_as_float(iq[I_DEPTH * c]), score);
         atomicAdd(&shits[si], 1u);
@@ -1654,6 +1694,8 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
           const V3<R> d{q9.b, q9.c, q9.d};
           cx<R> rm, rn;
           reverse_field(lc, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
+          pr_m = rm;
+          pr_n = rn;
           const R arg =
               lc.k0 * ((P.s.x + lc.ldir[0]) * d.x + (P.s.y + lc.ldir[1]) * d.y + (P.s.z + lc.ldir[2]) * d.z);
           R sa, ca;

@@ -153,6 +153,13 @@ struct lmc_scene {
   std::vector<double> aux_host;
   int est_lds_n = 0;
   bool has_est = false;
+  // PhotonRecordSensor records: a device list per sensor-kernel launch,
+  // gathered on the host; rec_per_crossing = most record sensors on one plane
+  int rec_per_crossing = 0;
+  double rec_k = 0.0;  // 2 pi / wavelength of the scene's laser
+  DBuf<double> rec;
+  DBuf<unsigned long long> rec_count;  // [0] written, [1] no room
+  std::vector<double> rec_host;
   DBuf<double> out;
   DBuf<unsigned long long> hits;
   DBuf<unsigned long long> counter;  // [0] work counter, [1..5] stats
@@ -421,6 +428,9 @@ int validate(const lmc_scene_desc* d) {
         if (nt < 1) return fail(LMC_ERR_INVALID, "statistics N_t must be >= 1");
         break;
       }
+      case LMC_SENSOR_PHOTON_RECORD:
+        if (s.estimator) return fail(LMC_ERR_INVALID, "a PhotonRecordSensor has no estimator mode");
+        break;  // no accumulators: its records go to the scene's record list
       default: return fail(LMC_ERR_INVALID, "unknown sensor kind");
     }
     if (need != s.out_len) return fail(LMC_ERR_INVALID, "sensor out_len does not match its grid");
@@ -504,6 +514,41 @@ int64_t records_per_photon(const SceneInfo& sc, int max_events) {
 // scene i, photon indices photon_offset + [0, budgets[i]) within each scene, in
 // launches over the concatenated index space; use_graph captures the launch
 // sequence once into a hipGraph and replays it while (budgets, run) repeat.
+// PhotonRecordSensor records of one launch: once the transport kernel has
+// filled the crossing queue, size the device list for every queued crossing
+// times the record sensors of one plane (so no record can miss a slot), and
+// clear its counters; after the sensor kernel, append the records to the
+// scene's host list with the wave number filled in.
+template <class R> int prepare_records(lmc_scene* s, KParams<R>& p) {
+  LMC_HIP(hipEventSynchronize(s->ev1));
+  unsigned long long nq = 0;
+  LMC_HIP(hipMemcpy(&nq, s->qcount.p, sizeof(nq), hipMemcpyDeviceToHost));
+  const int64_t need = std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(nq), p.q_cap)) * s->rec_per_crossing;
+  if (static_cast<int64_t>(s->rec.n) < need * LMC_RECORD_FIELDS) {
+    const int64_t cap = need + need / 4 + 1024;
+    if (s->rec.alloc(static_cast<size_t>(cap) * LMC_RECORD_FIELDS)) return fail(LMC_ERR_NOMEM, "PhotonRecordSensor list: hipMalloc failed");
+  }
+  p.rec = s->rec.p;
+  p.rec_cap = static_cast<int64_t>(s->rec.n / LMC_RECORD_FIELDS);
+  p.rec_count = s->rec_count.p;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s->stream, s->rec_count.p, 2);
+  LMC_HIP(hipGetLastError());
+  return LMC_OK;
+}
+
+int gather_records(lmc_scene* s) {
+  unsigned long long cnt[2];
+  LMC_HIP(hipMemcpy(cnt, s->rec_count.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  if (cnt[1] != 0) return fail(LMC_ERR_DEVICE, "PhotonRecordSensor list overflowed");
+  const size_t n = static_cast<size_t>(cnt[0]), base = s->rec_host.size();
+  if (n == 0) return LMC_OK;
+  s->rec_host.resize(base + n * LMC_RECORD_FIELDS);
+  LMC_HIP(hipMemcpy(s->rec_host.data() + base, s->rec.p, sizeof(double) * n * LMC_RECORD_FIELDS,
+                    hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) s->rec_host[base + i * LMC_RECORD_FIELDS + 7] = s->rec_k;
+  return LMC_OK;
+}
+
 template <class R, bool kXo, bool kAdj>
 int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph,
              lmc_stats* st) {
@@ -606,11 +651,13 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     return fail(LMC_ERR_INVALID, "the CBS estimator needs track_reverse_paths (reverse path of every forced scatter)");
   const bool abs_run = p.abs_on != 0;
   if (abs_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by forward runs");
+  const bool rec_run = s->rec_per_crossing > 0;
+  if (rec_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "PhotonRecordSensor records are kept by forward runs");
   const bool track = r->track_reverse_paths != 0;
   bool has_stats = false;
   for (const SensorDev& sd : s->sensors_host) has_stats = has_stats || sd.kind == LMC_SENSOR_STATISTICS;
   // kernels without the penetration depth only where nothing reads it
-  const int aux_mode = abs_run ? 2 : ((has_stats || est || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
+  const int aux_mode = abs_run ? 2 : ((has_stats || rec_run || est || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
   auto pick = [&](bool wide, bool lds_t) -> TransportFn<R> {
     return transport_kernel<R, kXo, kAdj>(aux_mode, est, wide, lds_t, batch, track);
   };
@@ -764,6 +811,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
       hipLaunchKernelGGL(kt, dim3(grid), dim3(block), lds, s->stream, p);
       LMC_HIP(hipGetLastError());
       LMC_HIP(hipEventRecord(s->ev1, s->stream));
+      if (rec_run && p.q_cap > 0 && (rc = prepare_records(s, p))) return rc;
       if (p.q_cap > 0) {
         hipLaunchKernelGGL(kr, dim3(kr_grid), dim3(256), lds_r, s->stream, p);
         LMC_HIP(hipGetLastError());
@@ -780,6 +828,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
         LMC_HIP(hipMemcpy(qc, s->qcount.p, sizeof(qc), hipMemcpyDeviceToHost));
         if (qc[1] != 0) return fail(LMC_ERR_DEVICE, "crossing record queue overflowed");
       }
+      if (rec_run && p.q_cap > 0 && (rc = gather_records(s))) return rc;
     }
   }
   if (st) {
@@ -892,6 +941,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
   sc.absorption = d->absorption;
   sc.n_sensors = d->n_sensors;
   sc.laser = d->laser;
+  if (s->scenes.empty()) s->rec_k = 2.0 * M_PI / d->laser.wavelength;
   const int64_t tbase = static_cast<int64_t>(s->tables.size());
   const int scene_idx = static_cast<int>(s->scenes.size());
   if (d->n_tables) s->tables.insert(s->tables.end(), d->tables, d->tables + d->n_tables);
@@ -929,6 +979,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
     P.count = 0;
     P.scene = scene_idx;
     bool all_absorb = true;
+    int n_rec_plane = 0;
     for (int i = 0; i < d->n_sensors; ++i) {
       const lmc_sensor_desc& sd = d->sensors[i];
       if ((sd.z < zs[pl] - 1e-9) || (zs[pl] < sd.z - 1e-9)) continue;
@@ -1012,7 +1063,12 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       s->sensors_host.push_back(o);
       s->sensor_scene.push_back(scene_idx);
       P.count++;
-      if (sd.kind == LMC_SENSOR_FARFIELD_CBS && !o.est) P.has_cbs = 1;
+      if (sd.kind == LMC_SENSOR_FARFIELD_CBS && !o.est) P.flags |= kPlaneCbs;
+      if (sd.kind == LMC_SENSOR_PHOTON_RECORD) {
+        if (!privatise) return fail(LMC_ERR_UNSUPPORTED, "PhotonRecordSensor records are kept by single-scene runs only");
+        P.flags |= kPlaneRec;
+        ++n_rec_plane;
+      }
       if (!o.absorb) all_absorb = false;
     }
     // a plane on or outside the stack's faces is crossed only by a photon
@@ -1024,6 +1080,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
     else
       sc.n_multi_planes++;
     s->planes_host.push_back(P);
+    s->rec_per_crossing = std::max(s->rec_per_crossing, n_rec_plane);
   }
   s->n_out += d->n_out;
   s->scenes.push_back(sc);
@@ -1037,6 +1094,8 @@ int finish_set(lmc_scene* s) {
   if (!rc) rc = s->hits.alloc(std::max<size_t>(1, s->sensors_host.size()));
   if (!rc) rc = s->counter.alloc(8);
   if (!rc) rc = s->qcount.alloc(2);
+  if (rc) return rc;
+  if (!rc && s->rec_per_crossing > 0) rc = s->rec_count.alloc(2);
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
@@ -1062,6 +1121,7 @@ int zero_set(lmc_scene* s) {
   LMC_HIP(hipMemsetAsync(s->hits.p, 0, sizeof(unsigned long long) * s->hits.n, s->stream));
   if (s->grad.p) LMC_HIP(hipMemsetAsync(s->grad.p, 0, sizeof(double) * s->grad.n, s->stream));
   LMC_HIP(hipStreamSynchronize(s->stream));
+  s->rec_host.clear();
   return LMC_OK;
 }
 
@@ -1172,6 +1232,16 @@ int lmc_scene_run(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) {
 int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits) {
   if (!s) return fail(LMC_ERR_INVALID, "null scene");
   return read_scene(s, 0, out, n, hits);
+}
+
+int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n_records) {
+  if (!s) return fail(LMC_ERR_INVALID, "null scene");
+  if (max_records < 0) return fail(LMC_ERR_INVALID, "max_records must be >= 0");
+  const int64_t n = static_cast<int64_t>(s->rec_host.size() / LMC_RECORD_FIELDS);
+  if (n_records) *n_records = n;
+  if (out && max_records > 0)
+    std::memcpy(out, s->rec_host.data(), sizeof(double) * LMC_RECORD_FIELDS * std::min(n, max_records));
+  return LMC_OK;
 }
 
 int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int64_t n_grad, lmc_stats* st) {

@@ -14,8 +14,8 @@ from .phase import (DrainePhaseFunction, HenyeyGreensteinPhaseFunction, MiePhase
                     SamplingTable, UniformPhaseFunction, form_factor, mie_amplitudes, mie_info)
 from .medium import Layer, MieMedium, MixtureLayer, RGDMedium, Sample, SampleLayer, ScatteringMedium
 from .sensors import (CrossingDirection, FarFieldCBSSensor, FarFieldFluenceSensor, Laser, LaserSource,
-                      PlanarFieldSensor, PlanarFluenceSensor, ReflectanceSensor, Sensor, SensorsGroup,
-                      StatisticsSensor,
+                      PhotonRecord, PhotonRecordSensor, PlanarFieldSensor, PlanarFluenceSensor,
+                      ReflectanceSensor, Sensor, SensorsGroup, StatisticsSensor,
                       TemporalProfile)
 from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFieldProcessed,
                       PlanarFluenceProcessed, StokesMatrixProcessed, StokesRadialProcessed,

@@ -35,6 +35,8 @@ SENSOR_PLANAR_FIELD = 3
 SENSOR_FARFIELD_FLUENCE = 4
 SENSOR_REFLECTANCE = 5
 SENSOR_STATISTICS = 6
+SENSOR_PHOTON_RECORD = 7
+RECORD_FIELDS = 34  # doubles per PhotonRecordSensor record (LMC_RECORD_FIELDS)
 
 PREC_FP32 = 0
 PREC_FP64 = 1
@@ -156,6 +158,7 @@ def lib():
     L.lmc_scene_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunDesc), ctypes.POINTER(Stats)]
     L.lmc_scene_read.argtypes = [ctypes.c_void_p, c_dp, c_int64, ctypes.POINTER(c_uint64)]
     L.lmc_scene_zero.argtypes = [ctypes.c_void_p]
+    L.lmc_scene_records.argtypes = [ctypes.c_void_p, c_dp, c_int64, ctypes.POINTER(c_int64)]
     L.lmc_scene_run_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunDesc), c_dp, c_int64,
                                         ctypes.POINTER(Stats)]
     L.lmc_batch_create.argtypes = [c_int32, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_void_p)]

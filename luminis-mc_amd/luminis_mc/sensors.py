@@ -421,6 +421,69 @@ class StatisticsSensor(Sensor):
                                  self.N_t, self.dt, self.t_max], dtype=np.float64)
 
 
+class PhotonRecord:
+    """One detected photon's state (photon.hpp:259-291)."""
+
+    __slots__ = ("events", "penetration_depth", "launch_time", "arrival_time", "opticalpath", "weight", "k",
+                 "position_first_scattering", "position_last_scattering", "position_detector", "direction", "m",
+                 "n", "polarization_forward", "polarization_reverse")
+
+    def __init__(self):
+        self.events = 0
+        self.penetration_depth = self.launch_time = self.arrival_time = self.opticalpath = 0.0
+        self.weight = self.k = 0.0
+        self.position_first_scattering = Vec3()
+        self.position_last_scattering = Vec3()
+        self.position_detector = Vec3()
+        self.direction = Vec3(0, 0, 1)
+        self.m = Vec3(1, 0, 0)
+        self.n = Vec3(0, 1, 0)
+        self.polarization_forward = CVec2(1, 0)
+        self.polarization_reverse = CVec2(0, 0)
+
+    @classmethod
+    def from_row(cls, r) -> "PhotonRecord":
+        """From one LMC_RECORD_FIELDS row of lmc_scene_records."""
+        p = cls()
+        p.events = int(r[1])
+        p.penetration_depth, p.launch_time, p.arrival_time, p.opticalpath, p.weight, p.k = (float(x) for x in r[2:8])
+        p.position_first_scattering = Vec3(*r[8:11])
+        p.position_last_scattering = Vec3(*r[11:14])
+        p.position_detector = Vec3(*r[14:17])
+        p.direction = Vec3(*r[17:20])
+        p.m = Vec3(*r[20:23])
+        p.n = Vec3(*r[23:26])
+        p.polarization_forward = CVec2(complex(r[26], r[27]), complex(r[28], r[29]))
+        p.polarization_reverse = CVec2(complex(r[30], r[31]), complex(r[32], r[33]))
+        return p
+
+
+class PhotonRecordSensor(Sensor):
+    """Keeps a PhotonRecord per accepted crossing (detector.cpp:427-485).
+    Records arrive in crossing-queue order, not photon order; ``records`` is
+    the same data as an (n, 34) array (columns as LMC_RECORD_FIELDS)."""
+
+    kind = _lib.SENSOR_PHOTON_RECORD
+
+    def __init__(self, z: float, absorb: bool = True):
+        super().__init__(z, absorb, False)
+        self.records = np.zeros((0, _lib.RECORD_FIELDS))
+
+    @property
+    def recorded_photons(self) -> list:
+        return [PhotonRecord.from_row(r) for r in self.records]
+
+    def add_records(self, rows: np.ndarray) -> None:
+        """merge_from (detector.cpp:455-461): append another copy's records."""
+        if len(rows):
+            self.records = np.concatenate([self.records, np.asarray(rows, dtype=np.float64)])
+
+    def clone(self):
+        c = super().clone()
+        c.records = np.zeros((0, _lib.RECORD_FIELDS))
+        return c
+
+
 class SensorsGroup:
     """Container; add_detector stores a clone and returns the internal copy
     (python_module.cpp:315-322, detector.cpp:166-176)."""

@@ -201,11 +201,18 @@ class SceneBuild:
             absorption._fill_desc(desc.absorption, self.abs_offset)
         self.desc = desc
 
-    def scatter_results(self, out: np.ndarray, hits: np.ndarray) -> None:
-        """Add a flat accumulator array into the sensors (merge_from)."""
+    def scatter_results(self, out: np.ndarray, hits: np.ndarray, records: np.ndarray | None = None) -> None:
+        """Add a flat accumulator array into the sensors (merge_from), and
+        PhotonRecordSensor records (rows of lmc_scene_records, column 0 the
+        sensor's index) into their sensors."""
         for s, (off, n), h in zip(self.sensors, self.offsets, hits):
             s.buffer += out[off:off + n]
             s.hits += int(h)
+        if records is not None and len(records):
+            idx = records[:, 0].astype(np.int64)
+            for i, s in enumerate(self.sensors):
+                if s.kind == _lib.SENSOR_PHOTON_RECORD:
+                    s.add_records(records[idx == i])
         if self.absorption is not None:
             self.absorption.buffer += out[self.abs_offset:self.abs_offset + self.absorption.buffer.size]
 
@@ -266,12 +273,24 @@ class GpuScene:
                    "lmc_scene_read")
         return out, hits
 
+    def records(self) -> np.ndarray:
+        """PhotonRecordSensor records kept since creation or zero(): an
+        (n, RECORD_FIELDS) array, column 0 the sensor's index."""
+        n = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.lmc_scene_records(self._h, None, 0, ctypes.byref(n)), "lmc_scene_records")
+        rows = np.zeros((n.value, _lib.RECORD_FIELDS))
+        if n.value:
+            _lib.check(L.lmc_scene_records(self._h, _lib.dptr(rows), n.value, ctypes.byref(n)), "lmc_scene_records")
+        return rows
+
     def zero(self) -> None:
         _lib.check(_lib.lib().lmc_scene_zero(self._h), "lmc_scene_zero")
 
     def merge_into_sensors(self) -> None:
         out, hits = self.read()
-        self.build.scatter_results(out, hits)
+        has_rec = any(s.kind == _lib.SENSOR_PHOTON_RECORD for s in self.build.sensors)
+        self.build.scatter_results(out, hits, self.records() if has_rec else None)
 
     def close(self) -> None:
         if self._h:

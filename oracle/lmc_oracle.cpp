@@ -335,6 +335,7 @@ struct Photon {
   V3 r1, rn;
   C2 Traw, Traw_buf;
   bool has_T_prev{false};
+  CV E_rev{cplx(0.0, 0.0), cplx(0.0, 0.0)};  // polarization_reverse (photon.hpp), set by CBS process_hit
 };
 
 // Absorption::record_absorption (src/absortion.cpp:47-77).
@@ -358,8 +359,12 @@ void record_absorption(const Scene& S, const Photon& ph, double d_weight, double
 struct Accum {
   std::vector<double> out;
   std::vector<uint64_t> hits;
+  std::vector<double> records;  // PhotonRecordSensor rows, LMC_RECORD_FIELDS each
   uint64_t events{0}, trials{0};
 };
+
+// Records of the last oracle_run (all threads, thread order).
+std::vector<double> g_records;
 
 // Sensor::check_conditions (detector.cpp:374-417).
 bool check_conditions(const lmc_sensor_desc& d, V3 hit, V3 dir, int cross_dir, int events) {
@@ -533,6 +538,7 @@ void cbs_process_hit(const Scene& S, const lmc_sensor_desc& d, Photon& ph, const
     return;
   }
   const CV Er_loc = reverse_field(S, ph);
+  ph.E_rev = Er_loc;
   const V3 s_in = ph.P0.row(2);
   const V3 qb = (s_out + s_in) * ph.k;
   const cplx path_phase = std::exp(cplx(0.0, dot(qb, ph.rn - ph.r1)));
@@ -785,6 +791,21 @@ void fffluence_process_hit(const lmc_sensor_desc& d, Photon& ph, const HitInfo& 
 }
 
 // StatisticsSensor::process_hit (detector.cpp:1703-1800): counts in the set
+
+// PhotonRecordSensor::process_hit (detector.cpp:463-485): one row of the
+// record layout (luminis_hip.h LMC_RECORD_FIELDS) per accepted crossing.
+void record_process_hit(int si, const Photon& ph, const HitInfo& info, Accum& acc) {
+  const V3 d = ph.P.row(2), m = ph.P.row(0), n = ph.P.row(1);
+  const double row[LMC_RECORD_FIELDS] = {
+      static_cast<double>(si), static_cast<double>(ph.events), ph.penetration_depth, ph.launch_time,
+      ph.launch_time + info.opticalpath_at_hit / ph.velocity, info.opticalpath_at_hit, ph.weight, ph.k,
+      ph.r1.x, ph.r1.y, ph.r1.z, ph.rn.x, ph.rn.y, ph.rn.z, info.point.x, info.point.y, info.point.z,
+      d.x, d.y, d.z, m.x, m.y, m.z, n.x, n.y, n.z,
+      ph.E.m.real(), ph.E.m.imag(), ph.E.n.real(), ph.E.n.imag(),
+      ph.E_rev.m.real(), ph.E_rev.m.imag(), ph.E_rev.n.real(), ph.E_rev.n.imag()};
+  acc.records.insert(acc.records.end(), row, row + LMC_RECORD_FIELDS);
+  acc.hits[si] += 1;
+}
 // histograms; events, theta, phi and depth also per time window.
 void stats_process_hit(const Sensor& sen, Photon& ph, const HitInfo& info, double* g, uint64_t& hits) {
   const std::vector<double>& a = sen.stat;
@@ -895,6 +916,9 @@ bool record_hit(const Scene& S, Photon& ph, Accum& acc) {
           break;
         case LMC_SENSOR_STATISTICS:
           stats_process_hit(S.sensors[si], ph, info, g, acc.hits[si]);
+          break;
+        case LMC_SENSOR_PHOTON_RECORD:
+          record_process_hit(si, ph, info, acc);
           break;
         default:
           break;
@@ -1390,6 +1414,8 @@ int oracle_run(const lmc_scene_desc* desc, uint64_t n_photons, uint64_t photon_o
       return LMC_ERR_INVALID;
     }
     uint64_t ev = 0, tr = 0;
+    g_records.clear();
+    for (auto& a : acc) g_records.insert(g_records.end(), a.records.begin(), a.records.end());
     for (auto& a : acc) {
       for (int64_t i = 0; i < S.n_out; ++i) out[i] += a.out[i];
       if (hits)
@@ -1404,6 +1430,16 @@ int oracle_run(const lmc_scene_desc* desc, uint64_t n_photons, uint64_t photon_o
     g_err = e.what();
     return LMC_ERR_INVALID;
   }
+}
+
+// PhotonRecordSensor rows of the last oracle_run: count into *n, up to
+// max_rows rows into out (may be null).
+int oracle_records(double* out, int64_t max_rows, int64_t* n) {
+  const int64_t rows = static_cast<int64_t>(g_records.size() / LMC_RECORD_FIELDS);
+  if (n) *n = rows;
+  if (out && max_rows > 0)
+    std::memcpy(out, g_records.data(), sizeof(double) * LMC_RECORD_FIELDS * std::min(rows, max_rows));
+  return LMC_OK;
 }
 
 // Mie amplitudes (MIEV0 convention) and efficiencies, independent series.

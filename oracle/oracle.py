@@ -48,6 +48,7 @@ def lib():
         L.oracle_mix_seed.restype = ctypes.c_uint64
         L.oracle_reverse_field.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                            _dp, _dp, _dp]
+        L.oracle_records.argtypes = [_dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         L.oracle_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32)]
         _lib = L
@@ -69,6 +70,18 @@ def run(scene_build, n_photons: int, seed: int, max_events: int = 1000, track: b
     if rc != 0:
         raise ValueError(L.oracle_last_error().decode())
     return out[:scene_build.n_out], hits, ev.value, tr.value
+
+
+def records() -> np.ndarray:
+    """PhotonRecordSensor rows of the last run() ((n, 34), column 0 the
+    sensor's index; layout of LMC_RECORD_FIELDS)."""
+    L = lib()
+    n = ctypes.c_int64(0)
+    L.oracle_records(None, 0, ctypes.byref(n))
+    rows = np.zeros((n.value, 34))
+    if n.value:
+        L.oracle_records(rows.ctypes.data_as(_dp), n.value, ctypes.byref(n))
+    return rows
 
 
 def mie(x: float, m: complex, mu):

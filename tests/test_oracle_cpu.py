@@ -376,3 +376,59 @@ def test_absorption_grid_conserves_weight():
     img.buffer[:] = out[b.abs_offset:b.abs_offset + ab.buffer.size]
     im = img.get_absorption_image(n, 0)
     assert (im.rows, im.cols) == (2 * ab.n_r, ab.n_z)
+
+
+def _record_scene(n_theta=20, theta_max=0.4):
+    """Semi-infinite Mie (r 0.5 um, mu_s 10, mu_a 0.1 /mm), circular point beam;
+    at z = 0: a CBS sensor (20 x 4 bins over 0.4 rad, not absorbing), then a
+    PhotonRecordSensor and a total-reflectance sensor (both absorbing)."""
+    import luminis_mc as lmc
+    import scenes
+    from luminis_mc.simulation import SceneBuild
+    m, ph = scenes.mie_medium(0.5e-3, 1.59, 1.33, 10.0, 0.1)
+    sample = lmc.Sample(1.33)
+    sample.add_layer(m, 0.0, math.inf)
+    laser = lmc.Laser(1 / math.sqrt(2), 1j / math.sqrt(2), scenes.WAVELENGTH, 0.0, lmc.LaserSource.Point)
+    g = lmc.SensorsGroup()
+    g.add_detector(lmc.FarFieldCBSSensor(theta_max, 2 * math.pi, 0.0, theta_max / n_theta, 2 * math.pi / 4, 0.0))
+    g.detectors[0].absorb_photons = False
+    g.add_detector(lmc.PhotonRecordSensor(0.0))
+    g.add_detector(lmc.ReflectanceSensor(0.0, absorb=True))
+    b = SceneBuild(sample, laser, g.detectors)
+    b._keep = (m, ph)
+    return b
+
+
+def test_photon_records_carry_the_hit_state():
+    """PhotonRecordSensor::process_hit (detector.cpp:463-485): one record per
+    exit with the photon's state.  Pins: count = hits; weight sum = the
+    reflectance sensor's total (same exits); hit point on the plane; frames
+    orthonormal, |E_f| = 1; arrival = launch + path; k = 2 pi / lambda;
+    r_1 = r_n for single scattering; and the CBS grid rebuilt from the records
+    equals the CBS sensor's grid, which checks the reverse Jones vector, the
+    positions and the frames together."""
+    import records_check as rc
+    import scenes
+    b = _record_scene()
+    n = 6000
+    out, hits, _, _ = oracle.run(b, n, 17)
+    rows = oracle.records()
+    assert rows.shape[0] == int(hits[1]) > 0.3 * n
+    assert np.all(rows[:, rc.SI] == 1)
+    refl = out[b.offsets[2][0]]
+    assert abs(rows[:, rc.W].sum() - refl) <= 1e-9 * refl
+    assert np.abs(rows[:, 16]).max() < 1e-9
+    fe, je = rc.frame_errors(rows)
+    assert fe < 1e-9 and je < 1e-9
+    assert np.allclose(rows[:, rc.ARR] - rows[:, rc.T0], rows[:, rc.OPL], atol=1e-12)
+    assert np.allclose(rows[:, rc.K], 2 * math.pi / scenes.WAVELENGTH)
+    single = rows[:, rc.EV] == 1
+    assert single.any() and np.allclose(rows[single][:, rc.R1], rows[single][:, rc.RN])
+    assert np.all(rows[:, rc.DEPTH] >= 0) and np.all(rows[:, rc.EV] >= 1)
+    off, cnt = b.offsets[0]
+    cbs = out[off:off + cnt].reshape(8, 20, 4)
+    rebuilt = rc.cbs_from_records(rows, 20, 4, 0.4 / 20, 2 * math.pi / 4)
+    assert np.abs(cbs).max() > 0
+    assert np.allclose(rebuilt, cbs, rtol=1e-9, atol=1e-9 * np.abs(cbs).max())
+    # the reverse field matters: coherent and incoherent grids differ
+    assert np.abs(cbs[0] - cbs[4]).max() > 1e-3 * np.abs(cbs[4]).max()
