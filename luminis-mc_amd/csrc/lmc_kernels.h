@@ -31,10 +31,11 @@ template <class R> struct LaunchConst {
 // pointers the event loop reads stay apart from those only the prologue
 // copies (layers, species, sensors, planes, laser constants).
 template <class R> struct KParams {
-  const R* cdf;
-  const uint16_t* guide;
-  const STab<R>* stab;
-  const R* sel;
+  // the phase and amplitude tables in one blob: theta CDFs (R), their guide
+  // tables (u16), S1/S2 nodes (STab<R>) and species-selection CDFs (R), each
+  // section 32-byte aligned; species and layers hold offsets from the blob's
+  // start in units of the element type, so one base address serves all four
+  const unsigned char* tab;
   double* out;
   unsigned long long* hits;
   unsigned long long* counter;
@@ -49,7 +50,7 @@ template <class R> struct KParams {
   uint64_t n_photons, offset;
   uint64_t local_offset;        // batched runs: first photon index within every scene
   int32_t n_layers, n_species, n_sensors, n_planes, n_scenes;
-  int32_t cdf_total, guide_total, stab_total, sel_total;
+  int32_t tab_bytes;
   int32_t lds_grid, max_events, claim, q_chunk;
   int32_t lds_grad;             // adjoint runs: privatised grids keep their gradient rows in LDS too
   uint32_t key0, key1, idx_hi;
@@ -1444,23 +1445,17 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     q = smem + H::end;
   }
   if constexpr (kLdsTables && !kBatch) {
-    R* tc = reinterpret_cast<R*>(carve(sizeof(R) * p.cdf_total));
-    uint16_t* tg = reinterpret_cast<uint16_t*>(carve(sizeof(uint16_t) * p.guide_total + 4));
-    STab<R>* ts = reinterpret_cast<STab<R>*>(carve(sizeof(STab<R>) * p.stab_total));
-    R* tsel = reinterpret_cast<R*>(carve(sizeof(R) * p.sel_total));
-    copy(tc, p.cdf, sizeof(R) * p.cdf_total);
-    for (int i = tid; i < p.guide_total; i += nt) tg[i] = p.guide[i];
-    copy(ts, p.stab, sizeof(STab<R>) * p.stab_total);
-    copy(tsel, p.sel, sizeof(R) * p.sel_total);
-    v.cdf = tc;
-    v.guide = tg;
-    v.stab = ts;
-    v.sel = tsel;
+    unsigned char* tb = carve(p.tab_bytes);
+    copy(tb, p.tab, p.tab_bytes);
+    v.cdf = reinterpret_cast<const R*>(tb);
+    v.guide = reinterpret_cast<const uint16_t*>(tb);
+    v.stab = reinterpret_cast<const STab<R>*>(tb);
+    v.sel = reinterpret_cast<const R*>(tb);
   } else {
-    v.cdf = p.cdf;
-    v.guide = p.guide;
-    v.stab = p.stab;
-    v.sel = p.sel;
+    v.cdf = reinterpret_cast<const R*>(p.tab);
+    v.guide = reinterpret_cast<const uint16_t*>(p.tab);
+    v.stab = reinterpret_cast<const STab<R>*>(p.tab);
+    v.sel = reinterpret_cast<const R*>(p.tab);
   }
   if constexpr (kEst) {  // the workgroup's copy of the estimator grids that fit in LDS
     double* eg = reinterpret_cast<double*>(carve(sizeof(double) * p.est_lds_n));
@@ -1693,7 +1688,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
           const cx<R> T[4] = {{q7.b, q7.c}, {q7.d, q8.a}, {q8.b, q8.c}, {q8.d, q9.a}};
           const V3<R> d{q9.b, q9.c, q9.d};
           cx<R> rm, rn;
-          reverse_field(lc, p.stab, p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
+          reverse_field(lc, reinterpret_cast<const STab<R>*>(p.tab), p.species[spk & 0xFFFF], p.species[spk >> 16], P1, Pq, P, T, rm, rn);
           pr_m = rm;
           pr_n = rn;
           const R arg =

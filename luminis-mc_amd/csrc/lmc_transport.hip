@@ -103,13 +103,10 @@ template <class R> struct DevTables {
   bool ready = false;
   DBuf<LayerDev<R>> layers;
   DBuf<SpeciesDev<R>> species;
-  DBuf<R> cdf, sel;
-  DBuf<uint16_t> guide;
-  DBuf<STab<R>> stab;
+  DBuf<unsigned char> tab;  // CDFs, guide tables, S1/S2 nodes, selection CDFs (KParams::tab)
   DBuf<LaunchConst<R>> lcs;
   DBuf<R> aux;
-  int cdf_total = 0, guide_total = 0, stab_total = 0, sel_total = 0;
-  size_t table_bytes = 0;
+  size_t tab_bytes = 0;
 };
 
 // Per-scene bookkeeping of a scene set.
@@ -346,27 +343,34 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     c.launch_layer = s->scenes[k].launch_layer;
   }
   int rc;
-  if ((rc = T.layers.upload(L.data(), L.size()))) return rc;
-  if ((rc = T.species.upload(sp.data(), sp.size()))) return rc;
   if ((rc = T.lcs.upload(lc.data(), lc.size()))) return rc;
   {
     std::vector<R> eg(s->aux_host.begin(), s->aux_host.end());
     if (eg.empty()) eg.push_back(R(0));
     if ((rc = T.aux.upload(eg.data(), eg.size()))) return rc;
   }
-  if (cdf.empty()) cdf.push_back(R(1));
-  if (guide.empty()) guide.assign(2, 0);
-  if (sel.empty()) sel.push_back(R(1));
-  if ((rc = T.cdf.upload(cdf.data(), cdf.size()))) return rc;
-  if ((rc = T.guide.upload(guide.data(), guide.size()))) return rc;
-  if ((rc = T.stab.upload(stab.data(), stab.size()))) return rc;
-  if ((rc = T.sel.upload(sel.data(), sel.size()))) return rc;
-  T.cdf_total = static_cast<int>(cdf.size());
-  T.guide_total = static_cast<int>(guide.size());
-  T.stab_total = static_cast<int>(stab.size());
-  T.sel_total = static_cast<int>(sel.size());
-  T.table_bytes = sizeof(R) * cdf.size() + sizeof(uint16_t) * guide.size() + 4 + sizeof(STab<R>) * stab.size() +
-                  sizeof(R) * sel.size() + 64;
+  // one blob, sections 32-byte aligned (an STab<double> is 32 bytes); the
+  // species' and layers' offsets move by their section's start
+  auto al32 = [](size_t b) { return (b + 31) & ~static_cast<size_t>(31); };
+  const size_t off_g = al32(sizeof(R) * cdf.size());
+  const size_t off_s = off_g + al32(sizeof(uint16_t) * guide.size() + 4);
+  const size_t off_l = off_s + al32(sizeof(STab<R>) * stab.size());
+  const size_t total = off_l + al32(sizeof(R) * sel.size() + 4);
+  if (total > 0x7FFFFFF0ull) return fail(LMC_ERR_INVALID, "tables too large");
+  for (SpeciesDev<R>& o : sp) {
+    o.guide_off += static_cast<int32_t>(off_g / sizeof(uint16_t));
+    o.s_off += static_cast<int32_t>(off_s / sizeof(STab<R>));
+  }
+  for (LayerDev<R>& o : L) o.sel_off += static_cast<int32_t>(off_l / sizeof(R));
+  if ((rc = T.layers.upload(L.data(), L.size()))) return rc;
+  if ((rc = T.species.upload(sp.data(), sp.size()))) return rc;
+  std::vector<unsigned char> blob(total, 0);
+  if (!cdf.empty()) std::memcpy(blob.data(), cdf.data(), sizeof(R) * cdf.size());
+  if (!guide.empty()) std::memcpy(blob.data() + off_g, guide.data(), sizeof(uint16_t) * guide.size());
+  if (!stab.empty()) std::memcpy(blob.data() + off_s, stab.data(), sizeof(STab<R>) * stab.size());
+  if (!sel.empty()) std::memcpy(blob.data() + off_l, sel.data(), sizeof(R) * sel.size());
+  if ((rc = T.tab.upload(blob.data(), blob.size()))) return rc;
+  T.tab_bytes = total;
   T.ready = true;
   return LMC_OK;
 }
@@ -558,10 +562,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   std::memset(&p, 0, sizeof(p));
   p.layers = T.layers.p;
   p.species = T.species.p;
-  p.cdf = T.cdf.p;
-  p.guide = T.guide.p;
-  p.stab = T.stab.p;
-  p.sel = T.sel.p;
+  p.tab = T.tab.p;
   p.sensors = s->sensors_dev.p;
   p.planes = s->planes_dev.p;
   p.lcs = T.lcs.p;
@@ -584,10 +585,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.n_sensors = static_cast<int32_t>(s->sensors_host.size());
   p.n_planes = static_cast<int32_t>(s->planes_host.size());
   p.n_scenes = n_sc;
-  p.cdf_total = T.cdf_total;
-  p.guide_total = T.guide_total;
-  p.stab_total = T.stab_total;
-  p.sel_total = T.sel_total;
+  p.tab_bytes = static_cast<int32_t>(T.tab_bytes);
   p.lds_grid = s->lds_grid;
   p.key0 = static_cast<uint32_t>(r->seed);
   p.key1 = static_cast<uint32_t>(r->seed >> 32);
@@ -630,9 +628,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   // photon-index prefix of a batch or, when they fit, the staged tables
   const size_t base_lds = batch ? LdsHdr<R>::common_end + al16(sizeof(uint64_t) * (n_sc + 1)) : LdsHdr<R>::end;
   if (p.n_planes > kMaxSensors && !batch) return fail(LMC_ERR_INVALID, "too many sensor planes");
-  const size_t tab_lds = ((sizeof(R) * T.cdf_total + 15) & ~15ull) +
-                         ((sizeof(uint16_t) * T.guide_total + 4 + 15) & ~15ull) +
-                         ((sizeof(STab<R>) * T.stab_total + 15) & ~15ull) + ((sizeof(R) * T.sel_total + 15) & ~15ull);
+  const size_t tab_lds = T.tab_bytes;
   // A batch keeps the tables in HBM but can stage the per-event scene arrays
   // (layers, species, planes of all its scenes) in LDS.
   const size_t scene_lds = batch ? al16(sizeof(LayerDev<R>) * p.n_layers) + al16(sizeof(SpeciesDev<R>) * p.n_species) +
