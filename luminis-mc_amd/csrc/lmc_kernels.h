@@ -353,6 +353,17 @@ __device__ __forceinline__ void reverse_prefix(const LaunchConst<R>& lc, const S
   // Stage B: Q T^T Q with Q = diag(1,-1): [[T00, -T10], [-T01, T11]]
   xm = cadd(cmul(T[0], ym), cscale(cmul(T[2], yn), R(-1)));
   xn = cadd(cscale(cmul(T[1], ym), R(-1)), cmul(T[3], yn));
+  if constexpr (sizeof(R) == 4) {
+    // the field's scale cancels in the suffix's normalisation; bringing it to
+    // one here keeps the suffix's squared norm inside FP32 range whatever the
+    // scale of T (renormalised every other event in FP32 runs) and of S
+    const R n2 = cnorm(xm) + cnorm(xn);
+    if (n2 > R(0)) {
+      const R inv = M<R>::rsqrt(n2);
+      xm = cscale(xm, inv);
+      xn = cscale(xn, inv);
+    }
+  }
 }
 
 // Stage C of the reverse field and its normalisation (reverse_field_suffix,
@@ -1055,7 +1066,8 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
 template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, int kAux = 1>
-__device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph) {
+__device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph,
+                                               bool renorm = true) {
   // kPart 0: the whole event; 1: the flight up to the species of the next
   // vertex (left in ph.cur_sp); 2: the scatter at that vertex.  Estimator
   // runs split the event so the whole wave can work on each vertex in between.
@@ -1265,18 +1277,29 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     ph.mq = Po.m;
     ph.sq = Po.s;
     if (evt >= 3) {
-      // commit the pending J_{n-1} = diag(b2, b1) [[bc, bs], [-bs, bc]]:  T <- J T, Frobenius-normalised
+      // commit the pending J_{n-1} = diag(b2, b1) [[bc, bs], [-bs, bc]]:  T <- J T, Frobenius-normalised.
+      // Only T's direction is used (the reverse field is normalised), so the
+      // FP32 Philox path normalises every other wave iteration (renorm, a
+      // wave-uniform flag): between normalisations |T| stays within one
+      // factor max|S1|,|S2| of one.  The check modes normalise every event.
       const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
       const cx<R> r1 = cadd(cscale(ph.T[1], ph.bc), cscale(ph.T[3], ph.bs));
       const cx<R> r2 = cadd(cscale(ph.T[0], -ph.bs), cscale(ph.T[2], ph.bc));
       const cx<R> r3 = cadd(cscale(ph.T[1], -ph.bs), cscale(ph.T[3], ph.bc));
       const cx<R> t0 = cmul(ph.b2, r0), t1 = cmul(ph.b2, r1), t2 = cmul(ph.b1, r2), t3 = cmul(ph.b1, r3);
-      const R fr2 = cnorm(t0) + cnorm(t1) + cnorm(t2) + cnorm(t3);
-      const R inv = fr2 > R(0) ? M<R>::rsqrt(fr2) : R(1);
-      ph.T[0] = cscale(t0, inv);
-      ph.T[1] = cscale(t1, inv);
-      ph.T[2] = cscale(t2, inv);
-      ph.T[3] = cscale(t3, inv);
+      if (kXo || sizeof(R) == 8 || renorm) {
+        const R fr2 = cnorm(t0) + cnorm(t1) + cnorm(t2) + cnorm(t3);
+        const R inv = fr2 > R(0) ? M<R>::rsqrt(fr2) : R(1);
+        ph.T[0] = cscale(t0, inv);
+        ph.T[1] = cscale(t1, inv);
+        ph.T[2] = cscale(t2, inv);
+        ph.T[3] = cscale(t3, inv);
+      } else {
+        ph.T[0] = t0;
+        ph.T[1] = t1;
+        ph.T[2] = t2;
+        ph.T[3] = t3;
+      }
     }
     ph.b2 = s2;
     ph.b1 = s1;
@@ -1472,7 +1495,9 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   bool exhausted = false;
   const int lane = __lane_id();
   const uint64_t below_mask = (1ull << lane) - 1ull;
+  uint32_t iter = 0;
   for (;;) {
+    const bool renorm = (++iter & 1u) == 0u;  // wave-uniform: T normalised every other iteration
     const uint64_t dead = __ballot(!ph.alive);
     if (dead != 0 && !exhausted) {
       if (pool_next >= pool_end) {
@@ -1519,13 +1544,13 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     }
     int32_t ev0 = ph.events;
     if constexpr (kEst) {
-      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAux>(p, v, ph);
+      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAux>(p, v, ph, renorm);
       // SensorsGroup::run_estimators (detector.cpp:272-281) with the whole wave
       for (int si = 0; si < p.n_sensors; ++si)
         if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_wave<R, kXo>(p, v, ph, si);
-      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAux>(p, v, ph);
+      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAux>(p, v, ph, renorm);
     } else if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAux>(p, v, ph);
+      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAux>(p, v, ph, renorm);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));

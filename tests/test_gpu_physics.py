@@ -65,3 +65,32 @@ def test_cbs_cone_peaks_at_backscatter(require_gpu, pol):
         eco = (G[0] + G[1]) / (G[4] + G[5])
         print("co-polarised eta first bins", np.round(eco[:8], 3))
         assert eco[0] > eta[0] and eco[0] > 1.5
+
+
+def test_cbs_cone_with_nanometre_scatterers(require_gpu):
+    """Scatterers of r = 0.5 nm have |S1|, |S2| ~ 1e-7: the FP32 path (interior
+    product renormalised every other event, reverse-field prefix brought to
+    unit scale) must still form the reverse field: the cone is present
+    (a zeroed reverse field would give coherent = incoherent)."""
+    n = 1.33
+    r = 0.5e-6
+    ph = lmc.RayleighDebyeEMCPhaseFunction(scenes.WAVELENGTH, r, 1.59, n, 2000, 0.0, math.pi)
+    m = lmc.RGDMedium(ph, r, 1.59, n, scenes.WAVELENGTH)
+    m.set_scattering_coefficient(100.0)
+    m.set_absorption_coefficient(0.0)
+    m.set_mean_free_path(1.0 / 100.0)
+    sample = lmc.Sample(n)
+    sample.add_layer(m, 0.0, math.inf)
+    laser = lmc.Laser(1.0, 0.0, scenes.WAVELENGTH, 0.0, lmc.LaserSource.Point)
+    group = lmc.SensorsGroup()
+    group.add_detector(lmc.FarFieldCBSSensor(0.1, 6.28, 0.0, 0.002, 6.28, 0.0))
+    sc = GpuScene(sample, laser, group.detectors)
+    c = SimConfig()
+    c.seed, c.track_reverse_paths = 13, True
+    sc.run(run_desc_from(c, n_photons=20_000_000))
+    acc, hits = sc.read()
+    sc.close()
+    G = acc.reshape(8, 50)
+    eta = G[0] / G[4]
+    print("nm scatterers: hits", int(hits[0]), "eta(S0) first bins", np.round(eta[:6], 3))
+    assert eta[0] > 1.3
