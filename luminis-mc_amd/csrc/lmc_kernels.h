@@ -29,7 +29,9 @@ template <class R> struct LaunchConst {
 // Field order matters: the backend merges neighbouring kernel-argument loads
 // into wide scalar loads whose registers live (or spill) together, so the
 // pointers the event loop reads stay apart from those only the prologue
-// copies (layers, species, sensors, planes, laser constants).
+// copies (layers, species, sensors, planes, laser constants), and the
+// per-event scalars apart from the per-photon refill keys (a spilled group is
+// reloaded whole: 8 lane reads where one value is needed).
 template <class R> struct KParams {
   // the phase and amplitude tables in one blob: theta CDFs (R), their guide
   // tables (u16), S1/S2 nodes (STab<R>) and species-selection CDFs (R), each
@@ -45,13 +47,14 @@ template <class R> struct KParams {
   double* qd;                   // FP64 optical path at the crossing
   int32_t* qi;
   unsigned long long* q_count;  // [0] slots handed out, [1] overflowed
+  int32_t max_events, q_chunk;  // read every event / every crossing: loaded apart from the refill keys
   int64_t q_cap;
   int64_t n_out;
   uint64_t n_photons, offset;
   uint64_t local_offset;        // batched runs: first photon index within every scene
   int32_t n_layers, n_species, n_sensors, n_planes, n_scenes;
   int32_t tab_bytes;
-  int32_t lds_grid, max_events, claim, q_chunk;
+  int32_t lds_grid, claim;
   int32_t lds_grad;             // adjoint runs: privatised grids keep their gradient rows in LDS too
   uint32_t key0, key1, idx_hi;
   // staged into LDS by the prologue (read in the loop only by batched runs)
