@@ -55,7 +55,10 @@ int g_num_cu = 0;
 size_t g_max_lds_block = 0;  // dynamic LDS one workgroup may use
 size_t g_max_lds_cu = 0;     // LDS of one CU
 
+std::mutex g_device_mu;  // first use from several host threads
+
 int ensure_device() {
+  std::lock_guard<std::mutex> lk(g_device_mu);
   if (g_device_init == 1) return LMC_OK;
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
