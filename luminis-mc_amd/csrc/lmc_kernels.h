@@ -1677,21 +1677,34 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         if (static_cast<int64_t>(slot) < p.rec_cap) {
           const Quad<R> q10 = get4(f, 10), q11 = get4(f, 11);
           double* o = p.rec + static_cast<int64_t>(slot) * LMC_RECORD_FIELDS;
-          const double vals[LMC_RECORD_FIELDS] = {
-              static_cast<double>(sd.id), static_cast<double>(events),
-              static_cast<double>(__int_as_float(iq[I_DEPTH * c])), static_cast<double>(q10.d),
-              static_cast<double>(arr), static_cast<double>(q11.d), static_cast<double>(w), 0.0,
-              static_cast<double>(q10.a), static_cast<double>(q10.b), static_cast<double>(q10.c),
-              static_cast<double>(q11.a), static_cast<double>(q11.b), static_cast<double>(q11.c),
-              static_cast<double>(hx), static_cast<double>(hy), p.planes[pl].z,
-              static_cast<double>(P.s.x), static_cast<double>(P.s.y), static_cast<double>(P.s.z),
-              static_cast<double>(P.m.x), static_cast<double>(P.m.y), static_cast<double>(P.m.z),
-              static_cast<double>(P.n.x), static_cast<double>(P.n.y), static_cast<double>(P.n.z),
-              static_cast<double>(Em.r), static_cast<double>(Em.i), static_cast<double>(En.r),
-              static_cast<double>(En.i), static_cast<double>(pr_m.r), static_cast<double>(pr_m.i),
-              static_cast<double>(pr_n.r), static_cast<double>(pr_n.i)};
-#pragma unroll
-          for (int j = 0; j < LMC_RECORD_FIELDS; ++j) o[j] = vals[j];
+          // field by field (layout of LMC_RECORD_FIELDS), so no 34-double
+          // array raises the kernel's register peak
+          auto put = [&](int j, double x) { o[j] = x; };
+          auto put3 = [&](int j, R x, R y, R z) {
+            o[j] = static_cast<double>(x);
+            o[j + 1] = static_cast<double>(y);
+            o[j + 2] = static_cast<double>(z);
+          };
+          put(0, static_cast<double>(sd.id));
+          put(1, static_cast<double>(events));
+          put(2, static_cast<double>(__int_as_float(iq[I_DEPTH * c])));
+          put(3, static_cast<double>(q10.d));
+          put(4, static_cast<double>(arr));
+          put(5, static_cast<double>(q11.d));
+          put(6, static_cast<double>(w));
+          put(7, 0.0);  // k: filled by the host
+          put3(8, q10.a, q10.b, q10.c);
+          put3(11, q11.a, q11.b, q11.c);
+          put(14, static_cast<double>(hx));
+          put(15, static_cast<double>(hy));
+          put(16, p.planes[pl].z);
+          put3(17, P.s.x, P.s.y, P.s.z);
+          put3(20, P.m.x, P.m.y, P.m.z);
+          put3(23, P.n.x, P.n.y, P.n.z);
+          put3(26, Em.r, Em.i, En.r);
+          put3(29, En.i, pr_m.r, pr_m.i);
+          put(32, static_cast<double>(pr_n.r));
+          put(33, static_cast<double>(pr_n.i));
         } else {
           atomicAdd(&p.rec_count[1], 1ull);
         }
