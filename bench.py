@@ -41,8 +41,8 @@ PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no pac
 # counters read for a device-to-device copy of known size (`bench.py
 # --copy-probe 256`): fetch x2.00, write x1.00.  Table reads are cache
 # resident; the writes are the crossing records handed to the sensor kernel.
-HBM_FETCH_B_PER_PHOTON = 700.6 * 1024 * 2.00 / 4e6
-HBM_WRITE_B_PER_PHOTON = 6.369e5 * 1024 * 1.00 / 4e6
+HBM_FETCH_B_PER_PHOTON = 705.4 * 1024 * 2.00 / 4e6
+HBM_WRITE_B_PER_PHOTON = 6.381e5 * 1024 * 1.00 / 4e6
 
 
 def metric_name():
