@@ -1582,15 +1582,16 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   }
 }
 
-// FP32 forward runs: at most 128 VGPRs, so 16 waves of a CU stay resident
-// (4 per SIMD); the adjoint's per-layer scores take 3 waves per SIMD.
+// FP32 runs: at most 128 VGPRs, so 16 waves of a CU stay resident (4 per
+// SIMD); the adjoint's per-layer scores then spill 6-10 dwords to scratch,
+// which costs less than a fourth of the waves (3 per SIMD ran 20 % slower).
 // kAux: per-photon outputs beyond the sensors' crossings.  1 tracks the
 // penetration depth (maximum z) that StatisticsSensor histograms; 2 also
 // records the Absorption grid at every vertex; 0 (FP32 Philox forward runs
 // without either) carries no trace of them.
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, int kAux = 1>
 __global__ void __launch_bounds__(512)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (kAdj ? 3 : 4) : 2)))
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))
 k_transport(KParams<R> p) {
   transport_body<R, kXo, kTrack, kLdsTables, kAdj, kBatch, false, kAux>(p);
 }
