@@ -681,11 +681,11 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   size_t lds = lds_tables ? need : base_lds;
   void (*kt_est)(KParams<R>) = nullptr;
   if (est) {
-    // estimator runs: three 256-thread workgroups per CU (the kernel's
+    // estimator runs: four 256-thread workgroups per CU (the kernel's
     // register budget), the estimator grids in LDS, the tables beside them
-    // when all three copies fit
+    // when all four copies fit
     const size_t est_need = base_lds + al16(sizeof(double) * s->est_lds_n);
-    lds_tables = 3 * (est_need + tab_lds) <= g_max_lds_cu && est_need + tab_lds <= g_max_lds_block;
+    lds_tables = 4 * (est_need + tab_lds) <= g_max_lds_cu && est_need + tab_lds <= g_max_lds_block;
     lds = lds_tables ? est_need + tab_lds : est_need;
     kt_est = pick(false, lds_tables);
     if (!kt_est) return fail(LMC_ERR_UNSUPPORTED, "no estimator kernel for this run");
