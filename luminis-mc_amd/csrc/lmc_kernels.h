@@ -19,6 +19,7 @@ namespace lmc {
 // wavenumber and the absolute index of the layer holding the launch point.
 template <class R> struct LaunchConst {
   double lpos_x, lpos_y, sigma, pulse, rep, t_off;
+  double k0d;  // vacuum wavenumber in FP64: the planar field's phase k0 * optical path
   R lpos_z;
   R ldir[3], lm[3], ln[3];
   R pol[4];
@@ -683,7 +684,7 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   const int64_t c = p.q_cap;
   R* f = p.qf + static_cast<int64_t>(slot) * (4 * kRecQ);
   double opl_hit = ph.opl;
-  if (corr > R(0)) opl_hit -= static_cast<double>(corr) * static_cast<double>(v.L[ph.layer].n);
+  if (corr > R(0)) opl_hit -= static_cast<double>(corr) * layer_n64(v.L[ph.layer]);
   put4(f, 0, hp.x, hp.y, arr, ph.w);
   put4(f, 1, ph.Em.r, ph.Em.i, ph.En.r, ph.En.i);
   put4(f, 2, ph.P.m.x, ph.P.m.y, ph.P.m.z, ph.P.n.x);
@@ -719,9 +720,14 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
 }
 
 // SensorsGroup::record_hit (detector.cpp:178-270) for the segment prev->pos:
-// planes within [min z, max z] in increasing z.
+// planes within [min z, max z] in increasing z.  from_iface: the segment
+// starts on the layer interface the photon has just crossed (FP32 runs place
+// it exactly there; the reference, and the FP64 build, nudge it 1e-12 past),
+// so a plane at that interface, already recorded by the sub-step that ended
+// there, is not recorded again.
 template <class R, bool kXo, bool kAdj, int kAux>
-__device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, V3<R> prev) {
+__device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, V3<R> prev,
+                                           bool from_iface) {
   const R z1 = prev.z, z2 = ph.pos.z;
   if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
   const R zmin = z1 < z2 ? z1 : z2, zmax = z1 < z2 ? z2 : z1;
@@ -729,7 +735,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   const int pl0 = v.L[ph.layer].pl_first, pl1 = pl0 + v.L[ph.layer].pl_count;
   for (int pl = pl0; pl < pl1; ++pl) {
     const R zp = static_cast<R>(v.planes[pl].z);
-    if (zp >= zmin && zp <= zmax)
+    if (zp >= zmin && zp <= zmax && !(from_iface && zp == z1))
       killed |= plane_hit<R, kXo, kAdj, kAux>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
   }
   return killed;
@@ -1095,6 +1101,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
     const V3<R> dir = ph.P.s;
     R new_z = ph.pos.z + dir.z * step;
+    bool on_iface = false;  // FP32: the photon stands exactly on the interface it has just crossed
     // interface crossings (simulation.cpp:317-383), with the layer index tracked
     // explicitly (the reference nudges z by 1e-12 and re-derives it)
     for (;;) {
@@ -1119,7 +1126,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       const R step_to = step * t;
       const R rem = (step - step_to) * v.L[ph.layer].mu_t;
       ph.gpl += step_to;
-      ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step_to);
+      ph.opl += layer_n64(v.L[ph.layer]) * static_cast<double>(step_to);
       add_path<R, kXo, kAdj>(ph, step_to);
       const V3<R> prev = ph.pos;
       ph.pos.x += dir.x * step_to;
@@ -1129,17 +1136,18 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
         ph.pos.z = zi + (up ? 1e-12 : -1e-12);
       else
         ph.pos.z = zi;
-      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev)) {
+      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev, on_iface)) {
         ph.alive = false;
         return 0;
       }
+      on_iface = sizeof(R) == 4;
       ph.layer += up ? 1 : -1;
       const R mt = v.L[ph.layer].mu_t;
       step = mt > R(0) ? rem / mt : rem;
       new_z = ph.pos.z + dir.z * step;
     }
     ph.gpl += step;
-    ph.opl += static_cast<double>(v.L[ph.layer].n) * static_cast<double>(step);
+    ph.opl += layer_n64(v.L[ph.layer]) * static_cast<double>(step);
     add_path<R, kXo, kAdj>(ph, step);
     {
       const V3<R> prev = ph.pos;
@@ -1149,7 +1157,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
         if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
       }
-      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev)) {
+      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev, on_iface)) {
         ph.alive = false;
         return 0;
       }
@@ -1757,7 +1765,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         }
       } else if (sd.kind == LMC_SENSOR_PLANAR_FIELD) {
         double sp, cp;
-        ::sincos(static_cast<double>(lc.k0) * p.qd[i], &sp, &cp);
+        ::sincos(lc.k0d * p.qd[i], &sp, &cp);
         const double wsd = ::sqrt(static_cast<double>(w));
         const double exr = static_cast<double>(Em.r * P.m.x + En.r * P.n.x);
         const double exi = static_cast<double>(Em.i * P.m.x + En.i * P.n.x);

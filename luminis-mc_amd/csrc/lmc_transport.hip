@@ -315,6 +315,7 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.mu_t = static_cast<R>(d.mu_t);
     o.albedo = static_cast<R>(d.mu_t > 0 ? d.mu_s / d.mu_t : 1.0);
     o.n = static_cast<R>(d.n);
+    o.n_lo = static_cast<R>(d.n - static_cast<double>(o.n));
     o.mu_s = static_cast<R>(d.mu_s);
     o.mu_a = static_cast<R>(d.mu_a);
     o.sp_first = d.species_first;
@@ -342,6 +343,7 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     }
     for (int j = 0; j < 4; ++j) c.pol[j] = static_cast<R>(Ls.pol[j]);
     c.k0 = static_cast<R>(2.0 * M_PI / Ls.wavelength);
+    c.k0d = 2.0 * M_PI / Ls.wavelength;
     c.sigma = Ls.sigma;
     c.source = Ls.source;
     c.temporal = Ls.temporal;

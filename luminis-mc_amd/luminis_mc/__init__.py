@@ -12,7 +12,8 @@ from .linalg import (CMatrix, CVec2, Matrix, Rng, Vec2, Vec3, calculate_rotation
 from .phase import (DrainePhaseFunction, HenyeyGreensteinPhaseFunction, MiePhaseFunction, PhaseFunction,
                     RayleighDebyeEMCPhaseFunction, RayleighDebyePhaseFunction, RayleighPhaseFunction,
                     SamplingTable, UniformPhaseFunction, form_factor, mie_amplitudes, mie_info)
-from .medium import Layer, MieMedium, MixtureLayer, RGDMedium, Sample, SampleLayer, ScatteringMedium
+from .medium import (Layer, MieMedium, MixtureLayer, RGDMedium, Sample, SampleLayer, ScatteringMedium,
+                     lognormal_mie_mixture)
 from .sensors import (CrossingDirection, FarFieldCBSSensor, FarFieldFluenceSensor, Laser, LaserSource,
                       PhotonRecord, PhotonRecordSensor, PlanarFieldSensor, PlanarFluenceSensor,
                       ReflectanceSensor, Sensor, SensorsGroup, StatisticsSensor,
@@ -24,6 +25,7 @@ from .results import (FarFieldCBSProcessed, FarFieldCBSRadialProcessed, PlanarFi
 from .simulation import (GpuBatch, GpuScene, SimConfig, device_info, run_simulation_batched, run_simulation_parallel,
                          validate_sim_config)
 from .absorption import Absorption
+from .photon import LogLevel, Photon, gaussian_distribution, get_log_level, set_log_level, uniform_distribution
 from .tuning import autotune
 from .adjoint import DifferentiableTransport, crn_central_difference
 

@@ -37,6 +37,9 @@ SENSOR_REFLECTANCE = 5
 SENSOR_STATISTICS = 6
 SENSOR_PHOTON_RECORD = 7
 RECORD_FIELDS = 34  # doubles per PhotonRecordSensor record (LMC_RECORD_FIELDS)
+MAX_LAYERS = 8    # per scene (lmc_scene_create refuses more)
+MAX_SPECIES = 64  # per scene
+MAX_SENSORS = 8   # per scene
 
 PREC_FP32 = 0
 PREC_FP64 = 1

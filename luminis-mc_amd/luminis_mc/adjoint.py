@@ -48,9 +48,21 @@ class _ParamBuild:
         self.n_out = base.n_out
         self.layers = (_lib.LayerDesc * nl)()
         ctypes.memmove(self.layers, base.layers, ctypes.sizeof(self.layers))
+        # A species shared by several layers (one medium object added to
+        # several layers) gets a copy per extra layer, so each layer's
+        # (mu_s, mu_a, g) stays its own; the copies share the tables.
         ns = base.desc.n_species
-        self.species = (_lib.SpeciesDesc * ns)()
-        ctypes.memmove(self.species, base.species, ctypes.sizeof(self.species))
+        first = [base.layers[l].species_first for l in range(nl)]
+        extra = [l for l in range(nl) if first.index(first[l]) != l]
+        if ns + len(extra) > _lib.MAX_SPECIES:
+            raise NotImplementedError(f"gradients: {ns + len(extra)} species after giving each layer its own "
+                                      f"(at most {_lib.MAX_SPECIES})")
+        self.species = (_lib.SpeciesDesc * (ns + len(extra)))()
+        ctypes.memmove(self.species, base.species, ctypes.sizeof(_lib.SpeciesDesc) * ns)
+        for k, l in enumerate(extra):
+            ctypes.memmove(ctypes.byref(self.species[ns + k]), ctypes.byref(base.species[first[l]]),
+                           ctypes.sizeof(_lib.SpeciesDesc))
+            self.layers[l].species_first = ns + k
         for l in range(nl):
             mu_s, mu_a, g = (float(v) for v in theta[l])
             if not (mu_s > 0.0 and mu_a >= 0.0):
@@ -68,6 +80,7 @@ class _ParamBuild:
         ctypes.memmove(ctypes.byref(d), ctypes.byref(base.desc), ctypes.sizeof(d))
         d.layers = ctypes.cast(self.layers, ctypes.POINTER(_lib.LayerDesc))
         d.species = ctypes.cast(self.species, ctypes.POINTER(_lib.SpeciesDesc))
+        d.n_species = ns + len(extra)
         self.desc = d
 
 

@@ -299,3 +299,48 @@ class Sample:
 
     def z_top(self) -> float:
         return self.layers[-1].z_max if self.layers else 0.0
+
+
+# ------------------------------------------------------ polydisperse media --
+def lognormal_mie_mixture(wavelength: float, median_radius: float, sigma_ln: float, n_particle: float,
+                          n_medium: float, n_bins: int, mu_s: float, mu_a: float, span: float = 3.0,
+                          n_div: int = 10000, n_table: int = 1000):
+    """A lognormal distribution of sphere radii as K Mie species for
+    ``Sample.add_mixture_layer`` (the MixtureLayer route of src/sample.cpp:41-104:
+    species drawn per vertex with probability n_i sigma_i / sum n_j sigma_j).
+
+    ln r is cut at median +- span * sigma_ln into n_bins bins of equal width in
+    ln r; bin k is one MieMedium at its geometric-mean radius, with number
+    density proportional to the lognormal probability of the bin, scaled so
+    the mixture's total scattering coefficient is mu_s (1/mm).  Every species
+    carries mu_a/mu_s of the mixture, so its absorption totals mu_a.  Each
+    species gets a MiePhaseFunction of n_div divisions over [0, pi] and
+    n_table-node S1/S2 tables.
+
+    Returns (species, number_densities, weights): weights are the bins'
+    lognormal probabilities, renormalised to sum to one."""
+    from .phase import MiePhaseFunction
+    K = int(n_bins)
+    if K < 1:
+        raise ValueError("lognormal_mie_mixture: n_bins must be >= 1")
+    if not (median_radius > 0.0 and sigma_ln > 0.0 and span > 0.0):
+        raise ValueError("lognormal_mie_mixture: need median_radius > 0, sigma_ln > 0 and span > 0")
+    if not (mu_s > 0.0 and mu_a >= 0.0):
+        raise ValueError("lognormal_mie_mixture: need mu_s > 0 and mu_a >= 0")
+    ln_med = math.log(median_radius)
+    edges = [ln_med - span * sigma_ln + 2.0 * span * sigma_ln * k / K for k in range(K + 1)]
+    cdf = [0.5 * (1.0 + math.erf((e - ln_med) / (sigma_ln * math.sqrt(2.0)))) for e in edges]
+    weights = [cdf[k + 1] - cdf[k] for k in range(K)]
+    tot = sum(weights)
+    weights = [w / tot for w in weights]
+    species = []
+    for k in range(K):
+        r = math.exp(0.5 * (edges[k] + edges[k + 1]))
+        ph = MiePhaseFunction(wavelength, r, n_particle, n_medium, int(n_div), 0.0, math.pi)
+        m = MieMedium(ph, r, n_particle, n_medium, wavelength, n_table=n_table)
+        m.set_scattering_coefficient(mu_s)
+        m.set_absorption_coefficient(mu_a)
+        m.set_mean_free_path(1.0 / (mu_s + mu_a))
+        species.append(m)
+    scale = mu_s / sum(w * m.scattering_cross_section() for w, m in zip(weights, species))
+    return species, [w * scale for w in weights], weights

@@ -127,19 +127,9 @@ def scene_polydisperse(K: int = 32, n_table: int = 2048, n_div: int = 10000, n_t
     CBS n_theta x n_phi over 0..theta_max and a PlanarFieldSensor field_n x field_n
     over field_len x field_len mm at z = 0.  Returns (sample, laser, sensors, keep)."""
     n = 1.33
-    mu_s_total, mu_a_total = 30.0, 0.002
-    ln_med, s_ln = math.log(0.4e-3), 0.3
-    edges = [ln_med - 3 * s_ln + 6 * s_ln * k / K for k in range(K + 1)]
-    phi = [0.5 * (1.0 + math.erf((e - ln_med) / (s_ln * math.sqrt(2.0)))) for e in edges]
-    species, weights, keep = [], [], []
-    for k in range(K):
-        r = math.exp(0.5 * (edges[k] + edges[k + 1]))
-        m, ph = mie_medium(r, 1.59, n, mu_s_total, mu_a_total, n_div=n_div, n_table=n_table)
-        species.append(m)
-        weights.append(phi[k + 1] - phi[k])
-        keep += [m, ph]
-    scale = mu_s_total / sum(w * m.scattering_cross_section() for w, m in zip(weights, species))
-    dens = [w * scale for w in weights]
+    species, dens, _ = lmc.lognormal_mie_mixture(WAVELENGTH, 0.4e-3, 0.3, 1.59, n, K, 30.0, 0.002, span=3.0,
+                                                 n_div=n_div, n_table=n_table)
+    keep = list(species)
     sample = lmc.Sample(n)
     sample.add_mixture_layer(species, dens, 0.0, math.inf)
     layer = sample.layers[0]

@@ -243,3 +243,108 @@ def test_azimuth_lattice_check_is_sensitive():
         bad = max(ac.cdf_discrepancy(f, *_azimuth_numpy(f, words, mistake)) for f in forms)
         assert bad > 5 * ac.CDF_TOL, mistake
 
+
+
+def test_param_build_gives_shared_medium_one_species_per_layer():
+    """Two layers built from one medium object share a species in the scene
+    build; the gradient build gives each layer its own copy so each layer's g
+    (and mu_s, mu_a) is its own."""
+    from luminis_mc.adjoint import _ParamBuild
+    ph = lmc.HenyeyGreensteinPhaseFunction(0.7)
+    m = lmc.RGDMedium(ph, 0.1e-3, 1.59, 1.33, 0.633e-3)
+    m.set_scattering_coefficient(10.0)
+    m.set_absorption_coefficient(0.01)
+    m.set_mean_free_path(1.0 / 10.01)
+    sample = lmc.Sample(1.33)
+    sample.add_layer(m, 0.0, 0.5)
+    sample.add_layer(m, 0.5, math.inf)
+    laser = lmc.Laser(1.0, 0.0, 0.633e-3, 0.0, lmc.LaserSource.Point)
+    group = lmc.SensorsGroup()
+    group.add_detector(lmc.ReflectanceSensor(0.0, absorb=False))
+    tr = lmc.DifferentiableTransport(sample, laser, group.detectors, n_photons=10)
+    assert tr.base.layers[0].species_first == tr.base.layers[1].species_first
+    th = np.array([[10.0, 0.01, 0.6], [12.0, 0.02, 0.8]])
+    b = _ParamBuild(tr.base, th)
+    s0, s1 = b.layers[0].species_first, b.layers[1].species_first
+    assert s0 != s1 and b.desc.n_species == tr.base.desc.n_species + 1
+    assert b.species[s0].hg_g == 0.6 and b.species[s1].hg_g == 0.8
+    assert b.species[s0].mu_s == 10.0 and b.species[s1].mu_s == 12.0
+    assert b.species[s1].s1_offset == b.species[s0].s1_offset  # tables shared
+    assert tr.base.species[s0].hg_g == 0.7  # the template is untouched
+
+
+def test_lognormal_mixture_weights_and_total_scattering():
+    """Bin weights are the lognormal probabilities of equal ln r bins over
+    +-span sigma (closed form by erf), renormalised; number densities give
+    the requested total mu_s and mu_a through the MixtureLayer sums."""
+    K, med, s_ln, span = 8, 0.4e-3, 0.3, 3.0
+    sp, dens, w = lmc.lognormal_mie_mixture(0.633e-3, med, s_ln, 1.59, 1.33, K, 30.0, 0.002, span=span,
+                                            n_div=400, n_table=200)
+    assert len(sp) == len(dens) == len(w) == K
+    assert sum(w) == pytest.approx(1.0, abs=1e-15)
+    # symmetric about the median in ln r, central bins heaviest
+    assert np.allclose(w, w[::-1], rtol=1e-12)
+    z = np.linspace(-span, span, K + 1)
+    from scipy.stats import norm as _norm
+    pk = np.diff(_norm.cdf(z))
+    assert np.allclose(w, pk / pk.sum(), rtol=1e-12)
+    radii = np.array([m.radius for m in sp])
+    assert np.allclose(np.log(radii), np.log(med) + 0.5 * (z[:-1] + z[1:]) * s_ln, rtol=0, atol=1e-12)
+    layer = lmc.MixtureLayer(sp, dens, 0.0, math.inf)
+    assert layer.mu_scattering() == pytest.approx(30.0, rel=1e-12)
+    assert layer.mu_absorption() == pytest.approx(0.002, rel=1e-12)
+    with pytest.raises(ValueError):
+        lmc.lognormal_mie_mixture(0.633e-3, med, 0.0, 1.59, 1.33, K, 30.0, 0.002)
+
+
+def test_reference_public_names_are_exported():
+    """Every transport-surface name of the reference package
+    (python/luminis_mc/__init__.py:1-76) imports from luminis_mc; the MCMC
+    mean-free-path helpers and the HDF5 manager are not on the transport path."""
+    names = """Rng Vec3 Vec2 CVec2 Matrix CMatrix dot cross norm matcmul matcmulscalar calculate_rotation_angle
+    PhaseFunction UniformPhaseFunction RayleighPhaseFunction HenyeyGreensteinPhaseFunction RayleighDebyePhaseFunction
+    RayleighDebyeEMCPhaseFunction DrainePhaseFunction MiePhaseFunction form_factor Photon PhotonRecord LaserSource
+    TemporalProfile Laser uniform_distribution gaussian_distribution Sensor PhotonRecordSensor PlanarFieldSensor
+    PlanarFluenceSensor FarFieldCBSSensor StatisticsSensor SensorsGroup CrossingDirection StokesMatrixProcessed
+    StokesRadialProcessed FarFieldCBSProcessed FarFieldCBSRadialProcessed PlanarFluenceProcessed
+    PlanarFieldProcessed postprocess_farfield_cbs postprocess_planar_fluence postprocess_planar_field
+    ScatteringMedium RGDMedium MieMedium Layer SampleLayer MixtureLayer Sample Absorption SimConfig
+    run_simulation_parallel LogLevel set_log_level""".split()
+    missing = [n for n in names if not hasattr(lmc, n)]
+    assert not missing, missing
+    assert [int(lmc.LogLevel.debug), int(lmc.LogLevel.info), int(lmc.LogLevel.warn), int(lmc.LogLevel.error),
+            int(lmc.LogLevel.off)] == [1, 2, 3, 4, 6]
+    lmc.set_log_level(lmc.LogLevel.warn)
+    assert lmc.get_log_level() == lmc.LogLevel.warn
+    lmc.set_log_level(lmc.LogLevel.info)
+
+
+def test_photon_and_source_distributions():
+    """Photon's constructor frame and Stokes vector (src/photon.cpp:24-90);
+    uniform_distribution / gaussian_distribution against the same draws taken
+    by hand from the same xoshiro stream (src/laser.cpp:11-27), and their
+    moments: disk radius^2 mean sigma^2/2, Gaussian per-axis variance sigma^2."""
+    V = lmc.Vec3
+    p = lmc.Photon(V(1, 2, 3), V(0, 0, 1), V(1, 0, 0), V(0, 1, 0), 0.5)
+    assert p.pos.as_tuple() == (1, 2, 3) and p.prev_pos.as_tuple() == (1, 2, 3)
+    assert p.k == pytest.approx(4 * math.pi)
+    assert np.array_equal(np.asarray(p.P_local), np.eye(3))
+    p.set_polarization(lmc.CVec2(1 / math.sqrt(2), 1j / math.sqrt(2)))
+    I, Q, U, Vs = p.get_stokes_parameters()
+    assert (I, Q, U) == pytest.approx((1.0, 0.0, 0.0), abs=1e-15) and Vs == pytest.approx(1.0)
+    p.polarized = False
+    assert p.get_stokes_parameters() == [1.0, 0.0, 0.0, 0.0]
+    r1, r2 = lmc.Rng(77), lmc.Rng(77)
+    c = V(0.5, -0.25, 7.0)
+    q = lmc.uniform_distribution(r1, c, 2.0)
+    th, rr = r2.uniform() * 2 * math.pi, math.sqrt(r2.uniform()) * 2.0
+    assert (q.x, q.y, q.z) == (0.5 + rr * math.cos(th), -0.25 + rr * math.sin(th), 7.0)
+    q = lmc.gaussian_distribution(r1, c, 3.0)
+    assert (q.x, q.y) == (r2.normal(0.5, 3.0), r2.normal(-0.25, 3.0))
+    rng = lmc.Rng(5)
+    n = 20000
+    d = np.array([lmc.uniform_distribution(rng, V(), 1.0).as_tuple() for _ in range(n)])
+    r2m = (d[:, 0] ** 2 + d[:, 1] ** 2).mean()
+    assert abs(r2m - 0.5) < 5 * math.sqrt(1 / 12 / n) and (d[:, 0] ** 2 + d[:, 1] ** 2).max() <= 1.0
+    g = np.array([lmc.gaussian_distribution(rng, V(), 2.0).as_tuple() for _ in range(n)])
+    assert abs(g[:, 0].var() - 4.0) < 5 * 4.0 * math.sqrt(2 / n) and abs(g[:, 1].mean()) < 5 * 2.0 / math.sqrt(n)
