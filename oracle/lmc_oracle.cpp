@@ -36,7 +36,12 @@
 // recurrences, in MIEV0's sign convention) and the SamplingTable CDF builder,
 // used to check the library's own host table builders.
 //
-// Built with -ffp-contract=off so its arithmetic does not depend on the host.
+// Built with -ffp-contract=off so its arithmetic does not depend on the host
+// (liblmc_oracle.so, the parity checker).  The same source also builds
+// liblmc_oracle_fast.so at release flags (the CPU baseline's timing build) and
+// liblmc_oracle_plant.so with -DLMC_ORACLE_PLANTS: there oracle_set_plant(k)
+// switches on one planted mistake (enum Plant) so the tests can show that each
+// of their checks fails for a plausible slip.
 
 #include <algorithm>
 #include <atomic>
@@ -57,6 +62,31 @@ namespace {
 
 using cplx = std::complex<double>;
 constexpr double kPi = 3.14159265358979323846;
+
+// Planted mistakes (sensitivity checks of the tests; plant build only).
+enum Plant {
+  P_NONE = 0,
+  P_XO_ROT,               // xoshiro256++ state rotation 44 instead of 45
+  P_TABLE_T,              // SamplingTable lerp fraction measured from the upper node
+  P_DATA_T,               // DataTable lerp fraction measured from the upper node
+  P_PHI_C_SIGN,           // azimuth density with the sin*cos term's sign flipped
+  P_HG_FRAC,              // HG inversion with the fraction not squared
+  P_RESCALE_OLD_MU,       // interface rescale with the old layer's mu_t
+  P_FRAME_TRANSPOSE,      // frame update with A^T
+  P_JONES_SWAP,           // Jones update with S1 and S2 exchanged
+  P_DISK_NO_SQRT,         // uniform disk source radius sigma*u instead of sigma*sqrt(u)
+  P_PULSE_FWHM_AS_SIGMA,  // Gaussian pulse: FWHM used as the standard deviation
+  P_FLUENCE_POS_NEG,      // planar fluence binning at -x
+  P_FLUENCE_NO_XNEG,      // planar fluence lab field without the x negation
+  P_FIELD_PHASE_GEOM,     // field phase from the geometric instead of the optical path
+  P_ISO_THETA,            // isotropic theta as acos(u) instead of acos(2u - 1)
+};
+#ifdef LMC_ORACLE_PLANTS
+int g_plant = P_NONE;
+#define PLANT(k) (g_plant == (k))
+#else
+#define PLANT(k) false
+#endif
 
 // ---------------------------------------------------------------- RNG ----
 // mix_seed: src/rng.cpp:5-10.  xoshiro256++ and SplitMix64 seeding:
@@ -90,7 +120,7 @@ struct Xoshiro {
     s[1] ^= s[2];
     s[0] ^= s[3];
     s[2] ^= t;
-    s[3] = rotl(s[3], 45);
+    s[3] = rotl(s[3], PLANT(P_XO_ROT) ? 44 : 45);
     return r;
   }
   double uniform() { return (next() >> 11) * 0x1.0p-53; }
@@ -205,7 +235,7 @@ double table_sample(const Species& sp, double u) {
   if (idx >= static_cast<long>(c.size())) return v0 + (c.size() - 1) * st;
   const double xa = v0 + (idx - 1) * st;
   const double xb = v0 + idx * st;
-  const double t = (u - c[idx - 1]) / (c[idx] - c[idx - 1]);
+  const double t = PLANT(P_TABLE_T) ? (c[idx] - u) / (c[idx] - c[idx - 1]) : (u - c[idx - 1]) / (c[idx] - c[idx - 1]);
   return xa + t * (xb - xa);
 }
 
@@ -216,7 +246,7 @@ cplx data_sample(const std::vector<cplx>& y, double inv_dx, double x) {
   const size_t last = y.size() - 1;
   if (pos >= static_cast<double>(last)) return y.back();
   const size_t i = static_cast<size_t>(pos);
-  const double t = pos - static_cast<double>(i);
+  const double t = PLANT(P_DATA_T) ? 1.0 - (pos - static_cast<double>(i)) : pos - static_cast<double>(i);
   return y[i] + t * (y[i + 1] - y[i]);
 }
 
@@ -231,10 +261,10 @@ double sample_theta(const Species& sp, double u) {
       const double g = sp.d.hg_g;
       const double gg = g * g;
       const double frac = (1.0 - gg) / (1.0 - g + 2.0 * g * u);
-      return std::acos((1.0 + gg - frac * frac) / (2.0 * g));
+      return std::acos((1.0 + gg - (PLANT(P_HG_FRAC) ? frac : frac * frac)) / (2.0 * g));
     }
     default:  // uniform, src/phase.cpp:78-83
-      return std::acos(2.0 * u - 1.0);
+      return std::acos(PLANT(P_ISO_THETA) ? u : 2.0 * u - 1.0);
   }
 }
 
@@ -724,14 +754,14 @@ void fluence_process_hit(const lmc_sensor_desc& d, Photon& ph, const HitInfo& in
     t_idx = static_cast<int>(arrival / dt) + 1;
     if (t_idx >= Nt) return;
   }
-  const int xi = static_cast<int>((info.point.x + 0.5 * len_x) / dx);
+  const int xi = static_cast<int>(((PLANT(P_FLUENCE_POS_NEG) ? -info.point.x : info.point.x) + 0.5 * len_x) / dx);
   const int yi = static_cast<int>((info.point.y + 0.5 * len_y) / dy);
   if (xi < 0 || xi >= Nx || yi < 0 || yi >= Ny) return;
   const double w_sqrt = std::sqrt(ph.weight);
   const cplx Em = ph.E.m * info.phase * w_sqrt;
   const cplx En = ph.E.n * info.phase * w_sqrt;
   const M3& P = ph.P;
-  const cplx Ex = (Em * P(0, 0) + En * P(1, 0)) * -1.0;
+  const cplx Ex = (Em * P(0, 0) + En * P(1, 0)) * (PLANT(P_FLUENCE_NO_XNEG) ? 1.0 : -1.0);
   const cplx Ey = (Em * P(0, 1) + En * P(1, 1));
   const double v[4] = {std::norm(Ex) + std::norm(Ey), std::norm(Ex) - std::norm(Ey),
                        2.0 * std::real(Ex * std::conj(Ey)), 2.0 * std::imag(Ex * std::conj(Ey))};
@@ -888,7 +918,7 @@ bool record_hit(const Scene& S, Photon& ph, Accum& acc) {
     }
     HitInfo info;
     info.point = hp;
-    info.phase = std::exp(cplx(0, ph.k * opl_corr));
+    info.phase = std::exp(cplx(0, ph.k * (PLANT(P_FIELD_PHASE_GEOM) ? gpl_hit : opl_corr)));
     info.dir = cdir;
     info.opticalpath_at_hit = gpl_hit;
     for (int si : S.plane_sensors[p]) {
@@ -935,7 +965,8 @@ void emit(const Scene& S, Xoshiro& rng, Photon& ph) {
   V3 pos{L.pos[0], L.pos[1], L.pos[2]};
   if (L.source == LMC_SOURCE_UNIFORM) {
     const double th = rng.uniform() * 2.0 * kPi;
-    const double r = std::sqrt(rng.uniform()) * L.sigma;
+    const double u = rng.uniform();
+    const double r = (PLANT(P_DISK_NO_SQRT) ? u : std::sqrt(u)) * L.sigma;
     pos = {L.pos[0] + r * std::cos(th), L.pos[1] + r * std::sin(th), L.pos[2]};
   } else if (L.source == LMC_SOURCE_GAUSSIAN) {
     const double gx = rng.normal(L.pos[0], L.sigma);
@@ -958,7 +989,7 @@ void emit(const Scene& S, Xoshiro& rng, Photon& ph) {
   double t = t0;
   switch (L.temporal) {
     case LMC_TIME_GAUSSIAN:
-      t = rng.normal(t0, L.pulse_duration * FWHM_TO_SIGMA * PS_TO_NS);
+      t = rng.normal(t0, L.pulse_duration * (PLANT(P_PULSE_FWHM_AS_SIGMA) ? 1.0 : FWHM_TO_SIGMA) * PS_TO_NS);
       break;
     case LMC_TIME_TOPHAT:
       t = t0 + (rng.uniform() - 0.5) * L.pulse_duration * PS_TO_NS;
@@ -976,6 +1007,77 @@ void emit(const Scene& S, Xoshiro& rng, Photon& ph) {
       break;
   }
   ph.launch_time = t;
+}
+
+// PhaseFunction::sample_phi_conditional (phase.cpp:27-49): phi by rejection
+// against F(phi) = a cos^2 + b sin^2 + c sin cos with the closed-form maximum
+// Fmax = (a+b)/2 + sqrt((a-b)^2 + c^2)/2, two draws per trial.
+double phi_rejection(cplx s2, cplx s1, const CV& E, Xoshiro& rng, uint64_t& trials) {
+  const double s2sq = std::norm(s2), s1sq = std::norm(s1);
+  const double e1sq = std::norm(E.m), e2sq = std::norm(E.n);
+  const double e12 = std::real(E.m * std::conj(E.n));
+  const double a = s2sq * e1sq + s1sq * e2sq;
+  const double b = s1sq * e1sq + s2sq * e2sq;
+  const double c = 2.0 * (s2sq - s1sq) * e12 * (PLANT(P_PHI_C_SIGN) ? -1.0 : 1.0);
+  const double diff = (a - b);
+  const double Fmax = 0.5 * (a + b) + 0.5 * std::sqrt(diff * diff + c * c);
+  while (true) {
+    const double p = 2.0 * kPi * rng.uniform();
+    const double cp = std::cos(p), spn = std::sin(p);
+    const double F = a * cp * cp + b * spn * spn + c * cp * spn;
+    const double Fc = std::max(F, 0.0);
+    trials++;
+    if (rng.uniform() * Fmax <= Fc) return p;
+  }
+}
+
+// One scattering of the frame and the Jones vector (simulation.cpp:438-492):
+// P <- A(theta, phi) P with A's rows m' = cos(theta) (cos phi m + sin phi n) -
+// sin(theta) s, n' = -sin phi m + cos phi n, s' = sin(theta) (cos phi m +
+// sin phi n) + cos(theta) s; E <- S R(phi) E / sqrt(F), S = diag(s2, s1),
+// R(phi) = [[cos phi, sin phi], [-sin phi, cos phi]], F = |S R(phi) E|^2.
+void scatter_step(M3& P, CV& E, double theta, double phi, cplx s2, cplx s1) {
+  const double ct = std::cos(theta), st = std::sin(theta);
+  const double cph = std::cos(phi), sph = std::sin(phi);
+  M3 A;
+  A(0, 0) = ct * cph;
+  A(0, 1) = ct * sph;
+  A(0, 2) = -st;
+  A(1, 0) = -sph;
+  A(1, 1) = cph;
+  A(1, 2) = 0;
+  A(2, 0) = st * cph;
+  A(2, 1) = st * sph;
+  A(2, 2) = ct;
+  if (PLANT(P_FRAME_TRANSPOSE)) {
+    M3 At;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) At(i, j) = A(j, i);
+    A = At;
+  }
+  P = matmul(A, P);
+  if (PLANT(P_JONES_SWAP)) std::swap(s1, s2);
+  C2 Sm;
+  Sm(0, 0) = s2;
+  Sm(0, 1) = 0;
+  Sm(1, 0) = 0;
+  Sm(1, 1) = s1;
+  C2 R;
+  R(0, 0) = cph;
+  R(0, 1) = sph;
+  R(1, 0) = -sph;
+  R(1, 1) = cph;
+  C2 T = matcmul(Sm, R);
+  const cplx Em = E.m, En = E.n;
+  const double Emm = std::norm(Em), Enn = std::norm(En);
+  const double s22 = std::norm(s2), s11 = std::norm(s1);
+  const double pc = cph * cph, ps = sph * sph;
+  const double F = Emm * (s22 * pc + s11 * ps) + Enn * (s22 * ps + s11 * pc) +
+                   2.0 * std::real(Em * std::conj(En)) * (s22 - s11) * sph * cph;
+  const double Finv = 1.0 / std::sqrt(F);
+  for (auto& v : T.a) v *= Finv;
+  E.m = (T(0, 0) * Em + T(0, 1) * En);
+  E.n = (T(1, 0) * Em + T(1, 1) * En);
 }
 
 // run_photon (simulation.cpp:255-617).
@@ -1023,8 +1125,9 @@ void run_photon(const Scene& S, Photon& ph, Xoshiro& rng, Accum& acc, bool track
           ph.alive = false;
           break;
         }
+        const size_t old_layer = ph.layer;
         ph.layer = nl;
-        const double new_mu_t = S.layers[nl].d.mu_t;
+        const double new_mu_t = S.layers[PLANT(P_RESCALE_OLD_MU) ? old_layer : nl].d.mu_t;
         step = new_mu_t > 0 ? rem_opt / new_mu_t : rem_opt;
         new_z = ph.pos.z + dz * step;
         has = next_interface(S, ph.pos.z, new_z, zi);
@@ -1069,63 +1172,10 @@ void run_photon(const Scene& S, Photon& ph, Xoshiro& rng, Accum& acc, bool track
     cplx s2, s1;
     smatrix(med, theta, s2, s1);
     // sample_phi_conditional (phase.cpp:27-49)
-    double phi;
-    {
-      const double s2sq = std::norm(s2), s1sq = std::norm(s1);
-      const double e1sq = std::norm(ph.E.m), e2sq = std::norm(ph.E.n);
-      const double e12 = std::real(ph.E.m * std::conj(ph.E.n));
-      const double a = s2sq * e1sq + s1sq * e2sq;
-      const double b = s1sq * e1sq + s2sq * e2sq;
-      const double c = 2.0 * (s2sq - s1sq) * e12;
-      const double diff = (a - b);
-      const double Fmax = 0.5 * (a + b) + 0.5 * std::sqrt(diff * diff + c * c);
-      while (true) {
-        const double p = 2.0 * kPi * rng.uniform();
-        const double cp = std::cos(p), spn = std::sin(p);
-        const double F = a * cp * cp + b * spn * spn + c * cp * spn;
-        const double Fc = std::max(F, 0.0);
-        acc.trials++;
-        if (rng.uniform() * Fmax <= Fc) {
-          phi = p;
-          break;
-        }
-      }
-    }
-    const double ct = std::cos(theta), st = std::sin(theta);
+    const double phi = phi_rejection(s2, s1, ph.E, rng, acc.trials);
     const double cph = std::cos(phi), sph = std::sin(phi);
-    M3 A;
-    A(0, 0) = ct * cph;
-    A(0, 1) = ct * sph;
-    A(0, 2) = -st;
-    A(1, 0) = -sph;
-    A(1, 1) = cph;
-    A(1, 2) = 0;
-    A(2, 0) = st * cph;
-    A(2, 1) = st * sph;
-    A(2, 2) = ct;
-    ph.P = matmul(A, ph.P);
-    {  // polarized Jones update (simulation.cpp:456-492)
-      C2 Sm;
-      Sm(0, 0) = s2;
-      Sm(0, 1) = 0;
-      Sm(1, 0) = 0;
-      Sm(1, 1) = s1;
-      C2 R;
-      R(0, 0) = cph;
-      R(0, 1) = sph;
-      R(1, 0) = -sph;
-      R(1, 1) = cph;
-      C2 T = matcmul(Sm, R);
-      const cplx Em = ph.E.m, En = ph.E.n;
-      const double Emm = std::norm(Em), Enn = std::norm(En);
-      const double s22 = std::norm(s2), s11 = std::norm(s1);
-      const double pc = cph * cph, ps = sph * sph;
-      const double F = Emm * (s22 * pc + s11 * ps) + Enn * (s22 * ps + s11 * pc) +
-                       2.0 * std::real(Em * std::conj(En)) * (s22 - s11) * sph * cph;
-      const double Finv = 1.0 / std::sqrt(F);
-      for (auto& v : T.a) v *= Finv;
-      ph.E.m = (T(0, 0) * Em + T(0, 1) * En);
-      ph.E.n = (T(1, 0) * Em + T(1, 1) * En);
+    scatter_step(ph.P, ph.E, theta, phi, s2, s1);
+    {
       if (track) {  // simulation.cpp:495-567
         C2 Tr;
         Tr(0, 0) = s2 * cph;
@@ -1555,6 +1605,114 @@ void oracle_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint3
   out[1] = c1;
   out[2] = c2;
   out[3] = c3;
+}
+
+
+// ---- test hooks on the restatement's building blocks ----------------------
+// Switch on planted mistake k (0: none).  Returns 1 in the plant build, 0 (and
+// does nothing) in the others.
+int oracle_set_plant(int k) {
+#ifdef LMC_ORACLE_PLANTS
+  g_plant = k;
+  return 1;
+#else
+  (void)k;
+  return 0;
+#endif
+}
+
+// xoshiro256++ from a raw state s_in[4]: n outputs, and the state after each
+// step (4n words) when states is non-null.
+void oracle_xoshiro_state_seq(const uint64_t* s_in, int n, uint64_t* outputs, uint64_t* states) {
+  Xoshiro r;
+  for (int i = 0; i < 4; ++i) r.s[i] = s_in[i];
+  for (int i = 0; i < n; ++i) {
+    outputs[i] = r.next();
+    if (states)
+      for (int j = 0; j < 4; ++j) states[4 * i + j] = r.s[j];
+  }
+}
+
+// SamplingTable::Sample on a CDF of n nodes x_i = min + i*step, at m uniforms.
+void oracle_table_sample(const double* cdf, int n, double min_val, double step, int m, const double* u,
+                         double* out) {
+  Species sp;
+  std::memset(&sp.d, 0, sizeof(sp.d));
+  sp.d.cdf_min = min_val;
+  sp.d.cdf_step = step;
+  sp.cdf.assign(cdf, cdf + n);
+  for (int i = 0; i < m; ++i) out[i] = table_sample(sp, u[i]);
+}
+
+// DataTable::Sample on the uniform grid x_i = i*dx of n complex values
+// (re/im interleaved), at m points; out re/im interleaved.
+void oracle_data_sample(const double* y, int n, double dx, int m, const double* x, double* out) {
+  std::vector<cplx> v(n);
+  for (int i = 0; i < n; ++i) v[i] = cplx(y[2 * i], y[2 * i + 1]);
+  for (int i = 0; i < m; ++i) {
+    const cplx r = data_sample(v, 1.0 / dx, x[i]);
+    out[2 * i] = r.real();
+    out[2 * i + 1] = r.imag();
+  }
+}
+
+// n azimuths of the rejection sampler for amplitudes (s2, s1) and Jones
+// vector E (re/im of s2, s1, E.m, E.n in amp[8]) on the xoshiro stream of
+// state seed `seed`; total trials into *trials.
+void oracle_phi_rejection(const double* amp, uint64_t seed, int n, double* phi, uint64_t* trials) {
+  Xoshiro r;
+  r.seed(seed);
+  const cplx s2(amp[0], amp[1]), s1(amp[2], amp[3]);
+  const CV E{cplx(amp[4], amp[5]), cplx(amp[6], amp[7])};
+  uint64_t t = 0;
+  for (int i = 0; i < n; ++i) phi[i] = phi_rejection(s2, s1, E, r, t);
+  if (trials) *trials = t;
+}
+
+// The theta sampler of a closed-form phase function (kind HG or UNIFORM;
+// g for HG) at m uniforms.
+void oracle_sample_theta(int kind, double g, int m, const double* u, double* out) {
+  Species sp;
+  std::memset(&sp.d, 0, sizeof(sp.d));
+  sp.d.phase_kind = kind;
+  sp.d.hg_g = g;
+  for (int i = 0; i < m; ++i) out[i] = sample_theta(sp, u[i]);
+}
+
+// One scatter_step: frame P (rows m, n, s; 9 doubles) and Jones vector E
+// (re/im of m, n) in place, with amplitudes s2, s1 (re/im each in amp[4]).
+void oracle_scatter_step(double* P, double* E, double theta, double phi, const double* amp) {
+  M3 M;
+  for (int i = 0; i < 9; ++i) M.a[i] = P[i];
+  CV e{cplx(E[0], E[1]), cplx(E[2], E[3])};
+  scatter_step(M, e, theta, phi, cplx(amp[0], amp[1]), cplx(amp[2], amp[3]));
+  for (int i = 0; i < 9; ++i) P[i] = M.a[i];
+  E[0] = e.m.real();
+  E[1] = e.m.imag();
+  E[2] = e.n.real();
+  E[3] = e.n.imag();
+}
+
+// Emission of photons [offset, offset + n) of a scene (laser.cpp:77-99 on
+// each photon's stream mix_seed(seed, index)): x, y, z, launch time each.
+int oracle_emit(const lmc_scene_desc* desc, uint64_t seed, uint64_t offset, int64_t n, double* out) {
+  try {
+    const Scene S = build_scene(desc);
+    Xoshiro rng;
+    for (int64_t i = 0; i < n; ++i) {
+      rng.seed(mix_seed(seed, offset + static_cast<uint64_t>(i)));
+      Photon ph;
+      emit(S, rng, ph);
+      out[4 * i] = ph.pos.x;
+      out[4 * i + 1] = ph.pos.y;
+      out[4 * i + 2] = ph.pos.z;
+      out[4 * i + 3] = ph.launch_time;
+    }
+    return LMC_OK;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return LMC_ERR_INVALID;
+  }
 }
 
 }  // extern "C"

@@ -112,3 +112,48 @@ def engine_period_is_maximal(slip=None, state=(0x12345678, 0x9ABCDEF0, 0x0F1E2D3
     if _powx(order, poly, 128) != 1:
         return False
     return all(_powx(order // q, poly, 128) != 1 for q in PRIMES_2_128_MINUS_1)
+
+
+# 2^256 - 1 = F0 F1 ... F7 (Fermat numbers); F5 = 641 * 6700417,
+# F6 = 274177 * 67280421310721, F7 = 59649589127497217 * 5704689200685129054721
+PRIMES_2_256_MINUS_1 = (3, 5, 17, 257, 641, 65537, 274177, 6700417, 67280421310721, 59649589127497217,
+                        5704689200685129054721)
+
+
+def is_probable_prime(n: int) -> bool:
+    """Miller-Rabin with the first 13 prime bases (deterministic below 3.3e24)."""
+    if n < 2:
+        return False
+    bases = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41)
+    for p in bases:
+        if n % p == 0:
+            return n == p
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for a in bases:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def sequence_is_primitive(bits, degree: int, primes) -> bool:
+    """True when the bit sequence obeys a linear recurrence whose minimal
+    polynomial has the given degree and is primitive over GF(2), i.e. the
+    generating linear state update has period 2^degree - 1."""
+    conn, L = _berlekamp_massey(list(bits))
+    if L != degree:
+        return False
+    poly = int(bin(conn)[2:].zfill(L + 1)[::-1], 2)
+    order = (1 << degree) - 1
+    if _powx(order, poly, degree) != 1:
+        return False
+    return all(_powx(order // q, poly, degree) != 1 for q in primes)
