@@ -73,8 +73,12 @@ def write() -> None:
         f.write("[\n" + ",\n".join(json.dumps(s) for s in _json_safe(generate())) + "\n]\n")
 
 
-def build_scene(spec: dict, n_div: int = 10000):
-    """(sample, laser, [cbs sensor], keep-alive objects) of one scene."""
+def build_scene(spec: dict, n_div: int = 10000, extra_sensors: bool = False):
+    """(sample, laser, sensors group, cbs sensor, keep-alive objects) of one
+    scene.  extra_sensors: also a total reflectance and an 18 x 9 far-field
+    fluence over the backward hemisphere (non-absorbing, beside the CBS
+    sensor), so a parity check sees most exiting photons, not only the few in
+    the CBS cone."""
     import luminis_mc as lmc
     from scenes import WAVELENGTH, cbs_sensor
     n = 1.33
@@ -97,6 +101,10 @@ def build_scene(spec: dict, n_div: int = 10000):
     laser = lmc.Laser(1.0, 0.0, WAVELENGTH, 0.0, lmc.LaserSource.Point)
     group = lmc.SensorsGroup()
     det = group.add_detector(cbs_sensor(100, 0.1, 1))
+    if extra_sensors:
+        group.add_detector(lmc.ReflectanceSensor(0.0, absorb=False))
+        group.add_detector(lmc.FarFieldFluenceSensor(math.pi / 2, 2 * math.pi, (math.pi / 2) / 18, 2 * math.pi / 9,
+                                                     absorb=False))
     return sample, laser, group, det, keep
 
 

@@ -14,12 +14,13 @@ import batch_scenes
 import oracle
 import scenes
 from luminis_mc.simulation import GpuBatch, GpuScene, SimConfig, run_desc_from
+from parity import assert_trajectory_parity
 
 pytestmark = pytest.mark.gpu
 
 
-def _batch(specs):
-    built = [batch_scenes.build_scene(s, n_div=4000) for s in specs]
+def _batch(specs, extra_sensors=False):
+    built = [batch_scenes.build_scene(s, n_div=4000, extra_sensors=extra_sensors) for s in specs]
     b = GpuBatch([(sa, la, g.detectors) for sa, la, g, _, _ in built])
     b._keep = built
     return b
@@ -38,24 +39,28 @@ def _rd(precision, rng, seed, offset=0, **kw):
 
 
 def test_batch_fp64_xoshiro_matches_oracle_per_scene(require_gpu):
+    """Each scene of a batch against the oracle's run of that scene alone:
+    equal event, trial and hit counts, accumulators to rounding.  Besides its
+    CBS sensor each scene has a reflectance and a far-field fluence sensor, so
+    every scene's check rests on more than a thousand detected photons."""
     specs = batch_scenes.load()[:6]
-    budgets = np.array([1500, 2500, 1000, 3000, 2000, 1200], dtype=np.uint64)
-    b = _batch(specs)
+    budgets = np.array([3000, 5000, 2500, 6000, 4000, 3500], dtype=np.uint64)
+    b = _batch(specs, extra_sensors=True)
     st = b.run(budgets, _rd("fp64", "xoshiro", seed=11))
     print("batch stats", st)
     assert st["photons"] == int(budgets.sum())
-    ev_cpu = 0
+    ev_cpu = tr_cpu = 0
     for k in range(len(specs)):
         gpu, hits = b.read(k)
-        cpu, chits, cev, _ = oracle.run(b.builds[k], int(budgets[k]), 11)
-        ev_cpu += cev
-        scale = max(np.abs(cpu).max(), 1e-300)
-        err = np.abs(gpu - cpu).max() / scale
-        print(f"scene {k}: layers={len(specs[k]['layers'])} hits gpu={int(hits[0])} cpu={int(chits[0])} "
-              f"max rel diff={err:.2e}")
-        assert abs(int(hits[0]) - int(chits[0])) <= 1
-        assert err < 2e-3
-    assert abs(int(st["events"]) - ev_cpu) <= 2000
+        cpu_run = oracle.run(b.builds[k], int(budgets[k]), 11)
+        ev_cpu += cpu_run[2]
+        tr_cpu += cpu_run[3]
+        assert int(cpu_run[1][1]) >= 1000, k  # photons reaching the reflectance sensor
+        # per-scene event counts are not separated by the batch: compare the totals below
+        one = {"events": cpu_run[2], "phi_trials": cpu_run[3]}
+        assert_trajectory_parity(one, gpu, hits, cpu_run, b.builds[k].offsets, ["cbs", "refl", "ff"],
+                                 label=f"scene {k} ({len(specs[k]['layers'])} layers)")
+    assert int(st["events"]) == ev_cpu and int(st["phi_trials"]) == tr_cpu
     b.close()
 
 

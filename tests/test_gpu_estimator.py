@@ -21,6 +21,7 @@ import oracle
 import scenes
 import stats
 from luminis_mc.simulation import GpuBatch, GpuScene, SimConfig, run_desc_from
+from parity import assert_trajectory_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -80,21 +81,14 @@ def _run_desc(cfg, precision, rng, n, off=0):
     return run_desc_from(cfg, n_photons=n, photon_offset=off)
 
 
-def _fp64_parity(cfg, n, seed, rtol=2e-3):
+def _fp64_parity(cfg, n, seed):
     sc = GpuScene(cfg.sample, cfg.laser, cfg.detector.detectors)
     st = sc.run(_run_desc(cfg, "fp64", "xoshiro", n))
     gpu, hits = sc.read()
-    cpu, chits, cev, _ = oracle.run(sc.build, n, seed)
-    print("events gpu/cpu", st["events"], cev, "hits", list(hits), list(chits))
-    assert abs(int(st["events"]) - cev) <= 0.001 * cev
-    for i, ((off, cnt), h, ch) in enumerate(zip(sc.build.offsets, hits, chits)):
-        g, c = gpu[off:off + cnt], cpu[off:off + cnt]
-        scale = max(np.abs(c).max(), 1e-300)
-        rel = np.max(np.abs(g - c)) / scale
-        print("sensor", i, "max rel diff", rel, "hits", int(h), int(ch))
-        assert np.abs(c).max() > 0, i
-        assert rel < rtol, i
-        assert abs(int(h) - int(ch)) <= max(2, 1e-3 * int(ch)), i
+    cpu_run = oracle.run(sc.build, n, seed)
+    for i, (off, cnt) in enumerate(sc.build.offsets):
+        assert np.abs(cpu_run[0][off:off + cnt]).max() > 0, i
+    assert_trajectory_parity(st, gpu, hits, cpu_run, sc.build.offsets, label="estimator")
     sc.close()
     return gpu, sc.build
 

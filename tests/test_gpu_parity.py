@@ -21,6 +21,7 @@ import scenes
 import stats
 from luminis_mc import _lib
 from luminis_mc.simulation import GpuScene, SceneBuild, run_desc_from
+from parity import assert_trajectory_parity
 from test_oracle_cpu import PHILOX_KAT
 
 pytestmark = pytest.mark.gpu
@@ -97,24 +98,13 @@ def test_gpu_stream_uniformity(require_gpu):
     assert abs(r) < 5 / np.sqrt(n), r
 
 
-def _compare_exact(gpu, cpu, sensors_offsets, rtol):
-    for (off, n) in sensors_offsets:
-        g, c = gpu[off:off + n], cpu[off:off + n]
-        scale = max(np.abs(c).max(), 1e-300)
-        assert np.max(np.abs(g - c)) <= rtol * scale, (off, np.max(np.abs(g - c)) / scale)
-
-
 def test_gpu_fp64_xoshiro_trajectory_parity_plain_mie(require_gpu):
     cfg, det = scenes.config_plain_mie(20000, seed=3)
     sc = _scene(cfg)
     st = sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
     gpu, hits = sc.read()
-    cpu, chits, cev, ctr = oracle.run(sc.build, 20000, 3)
-    print("events gpu/cpu", st["events"], cev, "trials", st["phi_trials"], ctr, "hits", hits, chits)
     assert st["photons"] == 20000
-    assert abs(int(st["events"]) - cev) <= 1000  # a handful of photons may branch differently
-    assert int(hits[0]) == int(chits[0]) or abs(int(hits[0]) - int(chits[0])) <= 2
-    _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
+    assert_trajectory_parity(st, gpu, hits, oracle.run(sc.build, 20000, 3), sc.build.offsets, label="plain")
 
 
 def _time_resolved_scene(n_photons, seed):
@@ -143,8 +133,8 @@ def test_gpu_fp64_xoshiro_parity_time_resolved(require_gpu):
     sc = _scene(cfg)
     st = sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
     gpu, hits = sc.read()
-    cpu, chits, cev, _ = oracle.run(sc.build, 20000, 5)
-    print("events gpu/cpu", st["events"], cev, "hits", list(hits), list(chits))
+    cpu_run = oracle.run(sc.build, 20000, 5)
+    cpu = cpu_run[0]
     (o_cbs, n_cbs), (o_flu, n_flu) = sc.build.offsets[:2]
     n_t = n_cbs // (8 * 16 * 4)
     assert n_t == n_flu // (4 * 16 * 16) and n_t > 2
@@ -152,7 +142,7 @@ def test_gpu_fp64_xoshiro_parity_time_resolved(require_gpu):
         assert _bin0_minus_sum(arr[o_cbs:o_cbs + n_cbs], 8, n_t) < 1e-9
         assert _bin0_minus_sum(arr[o_flu:o_flu + n_flu], 4, n_t) < 1e-9
     assert np.abs(gpu[o_cbs:o_cbs + n_cbs]).max() > 0 and np.abs(gpu[o_flu:o_flu + n_flu]).max() > 0
-    _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
+    assert_trajectory_parity(st, gpu, hits, cpu_run, sc.build.offsets, ["cbs", "fluence"], label="time-resolved")
     # a second run accumulates: bin 0 of the read-out stays the sum of the time bins
     sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000, off=20000))
     gpu2, _ = sc.read()
@@ -166,15 +156,8 @@ def test_gpu_fp64_xoshiro_trajectory_parity_multilayer(require_gpu):
     sc = _scene(cfg)
     st = sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
     gpu, hits = sc.read()
-    cpu, chits, cev, ctr = oracle.run(sc.build, 20000, 11)
-    print("events gpu/cpu", st["events"], cev, "hits", list(hits), list(chits))
-    assert abs(int(st["events"]) - cev) <= 0.001 * cev
-    for (off, n), name in zip(sc.build.offsets, ["cbs", "fluence", "ff", "refl", "field"]):
-        g, c = gpu[off:off + n], cpu[off:off + n]
-        scale = max(np.abs(c).max(), 1e-300)
-        rel = np.max(np.abs(g - c)) / scale
-        print(name, "max rel diff", rel)
-        assert rel < 5e-3, name
+    assert_trajectory_parity(st, gpu, hits, oracle.run(sc.build, 20000, 11), sc.build.offsets,
+                             ["cbs", "fluence", "ff", "refl", "field"], label="multilayer")
 
 
 def test_gpu_fp32_statistical_parity_plain_mie(require_gpu):
@@ -260,10 +243,9 @@ def test_gpu_fp64_xoshiro_parity_steady_fluence_first(require_gpu):
     group.add_detector(lmc.FarFieldCBSSensor(0.1, 6.28, 0.0, 0.1 / 20, 6.28, 0.0))
     cfg.detector = group
     sc = _scene(cfg)
-    sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
+    st = sc.run(_run_desc(cfg, "fp64", "xoshiro", 20000))
     gpu, hits = sc.read()
-    cpu, chits, _, _ = oracle.run(sc.build, 20000, 13)
-    print("hits", list(hits), list(chits))
-    assert np.abs(cpu[:sc.build.offsets[0][1]]).max() > 0
-    _compare_exact(gpu, cpu, sc.build.offsets, rtol=2e-3)
+    cpu_run = oracle.run(sc.build, 20000, 13)
+    assert np.abs(cpu_run[0][:sc.build.offsets[0][1]]).max() > 0
+    assert_trajectory_parity(st, gpu, hits, cpu_run, sc.build.offsets, ["fluence", "cbs"], label="steady")
     sc.close()

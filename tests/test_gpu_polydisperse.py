@@ -16,6 +16,7 @@ import oracle
 import scenes
 import stats
 from luminis_mc.simulation import GpuScene, SimConfig, run_desc_from
+from parity import assert_trajectory_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -36,15 +37,8 @@ def test_polydisperse_fp64_xoshiro_trajectory_parity(require_gpu):
     n = 3000
     st = sc.run(_rd("fp64", "xoshiro", n, seed=5))
     gpu, hits = sc.read()
-    cpu, chits, cev, _ = oracle.run(sc.build, n, 5)
-    print("events gpu/cpu", st["events"], cev, "hits", hits, chits)
-    assert abs(int(st["events"]) - cev) <= 2000
-    for (off, m) in sc.build.offsets:
-        g, c = gpu[off:off + m], cpu[off:off + m]
-        scale = max(np.abs(c).max(), 1e-300)
-        err = np.abs(g - c).max() / scale
-        print("sensor at", off, "max rel diff", err)
-        assert err < 5e-3
+    assert_trajectory_parity(st, gpu, hits, oracle.run(sc.build, n, 5), sc.build.offsets, ["cbs", "field"],
+                             label="polydisperse")
     sc.close()
 
 

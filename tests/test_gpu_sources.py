@@ -18,6 +18,7 @@ import luminis_mc as lmc
 import oracle
 import scenes
 from luminis_mc.simulation import GpuScene, SimConfig, run_desc_from
+from parity import assert_trajectory_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -35,21 +36,14 @@ def _scene_with(source, profile, pulse_ps=0.0, rep_hz=0.0, offset_ps=0.0):
     return cfg
 
 
-def _parity(cfg, n, seed, max_events=1000, rtol=2e-3):
+def _parity(cfg, n, seed, max_events=1000):
     sc = GpuScene(cfg.sample, cfg.laser, cfg.detector.detectors)
     cfg.precision, cfg.rng, cfg.seed, cfg.MAX_EVENTS = "fp64", "xoshiro", seed, max_events
     st = sc.run(run_desc_from(cfg, n_photons=n))
     gpu, hits = sc.read()
-    cpu, chits, cev, _ = oracle.run(sc.build, n, seed, max_events=max_events)
-    print("events gpu/cpu", st["events"], cev, "hits", list(hits), list(chits))
-    assert abs(int(st["events"]) - cev) <= 0.002 * cev + 10
-    assert np.abs(cpu).max() > 0
-    for i, (off, cnt) in enumerate(sc.build.offsets):
-        g, c = gpu[off:off + cnt], cpu[off:off + cnt]
-        scale = max(np.abs(c).max(), 1e-300)
-        rel = np.abs(g - c).max() / scale
-        print("sensor", i, "max rel diff", rel)
-        assert rel < rtol if np.abs(c).max() > 0 else not g.any(), i
+    cpu_run = oracle.run(sc.build, n, seed, max_events=max_events)
+    assert np.abs(cpu_run[0]).max() > 0
+    assert_trajectory_parity(st, gpu, hits, cpu_run, sc.build.offsets, label="sources")
     sc.close()
     return gpu, sc.build
 
