@@ -287,6 +287,21 @@ void lmc_scene_destroy(lmc_scene* s);
  * sensors' process_hit (:546-568, :780-841, :1231-1310). */
 int lmc_scene_run(lmc_scene* s, const lmc_run_desc* run, lmc_stats* stats);
 
+/* lmc_scene_run without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * crossing queue is a static 64 MiB device array (one per device, held until
+ * the process ends) instead of a per-scene hipMalloc'd buffer, so runs split
+ * into launches of ~2e5 photons (FP32, one sensor plane).  Returns
+ * LMC_ERR_INVALID when one photon's worst-case crossing records do not fit
+ * the array beside the per-wave reserve, or when the run's precision has no
+ * device tables yet (FP32 tables come with lmc_scene_create, FP64 tables with
+ * a first lmc_scene_run); LMC_ERR_UNSUPPORTED for scenes with a
+ * PhotonRecordSensor.  Calls are serialised within the process (they share
+ * the array); the array keeps the last call's records, and each launch clears
+ * the slots it can use first. */
+int lmc_scene_run_noalloc(lmc_scene* s, const lmc_run_desc* run, lmc_stats* stats);
+
 /* Add the device accumulators into host buffers (n doubles at `out`,
  * layout of lmc_sensor_desc) and the per-sensor hit counters into `hits`
  * (n_sensors entries); either may be NULL.

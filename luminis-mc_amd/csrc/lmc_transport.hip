@@ -494,6 +494,18 @@ template <class K> int raise_lds_limit(K kfn, size_t lds) {
 constexpr int64_t kQueueRecords = int64_t(1) << 26;
 constexpr int kWaveQueueChunk = 128;  // queue slots a wave takes from the global counter at a time
 
+// Crossing queue of lmc_scene_run_noalloc: a file-scope device array instead
+// of a hipMalloc'd buffer, for callers that cannot allocate.  64 MiB holds
+// 284k FP32 records (236 B each) or 151k FP64 ones; waves take 16 slots at a
+// time there (a smaller per-wave reserve: 16 slots x 16 waves x 256 CUs), so a
+// launch carries ~219k FP32 photons of a scene with one sensor plane.  Each
+// device holds its own copy for the life of the process.  hipGetSymbolAddress
+// gives its device address once per device (g_scratch_ptr); host code uses
+// only that address, never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchQueueChunk = 16;
+constexpr int kMaxDevices = 64;
+
 void drop_graph(lmc_scene* s) {
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
   s->graph = nullptr;
@@ -563,10 +575,15 @@ int gather_records(lmc_scene* s) {
   return LMC_OK;
 }
 
+// scratch != nullptr: the no-allocation run (lmc_scene_run_noalloc): the
+// crossing queue lives at `scratch` (the device address of the static
+// array, kScratchBytes), nothing is allocated, PhotonRecordSensor records
+// (a host-gathered list) are refused.
 template <class R, bool kXo, bool kAdj>
 int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph,
-             lmc_stats* st) {
+             lmc_stats* st, unsigned char* scratch = nullptr) {
   const bool batch = budgets != nullptr;
+  if (scratch && (batch || use_graph)) return fail(LMC_ERR_INVALID, "no-alloc runs take one scene, eagerly");
   const int n_sc = static_cast<int>(s->scenes.size());
   KParams<R> p;
   std::memset(&p, 0, sizeof(p));
@@ -661,6 +678,9 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   if (abs_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by forward runs");
   const bool rec_run = s->rec_per_crossing > 0;
   if (rec_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "PhotonRecordSensor records are kept by forward runs");
+  if (rec_run && scratch)
+    return fail(LMC_ERR_UNSUPPORTED, "no-alloc runs keep no PhotonRecordSensor list (it is sized per launch): "
+                                     "use lmc_scene_run");
   const bool track = r->track_reverse_paths != 0;
   bool has_stats = false;
   for (const SensorDev& sd : s->sensors_host) has_stats = has_stats || sd.kind == LMC_SENSOR_STATISTICS;
@@ -701,10 +721,23 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   const int grid = g_num_cu * bpc;
   if (lds > g_max_lds_block) return fail(LMC_ERR_UNSUPPORTED, "scene constants exceed LDS");
   int64_t chunk = static_cast<int64_t>(total);
-  p.q_chunk = kWaveQueueChunk;
+  p.q_chunk = scratch ? kScratchQueueChunk : kWaveQueueChunk;
   // slots a launch can leave unused: the rest of every wave's last chunk
-  const int64_t reserve = static_cast<int64_t>(grid) * ((block + 63) / 64) * kWaveQueueChunk;
-  if (per_photon > 0) {
+  const int64_t reserve = static_cast<int64_t>(grid) * ((block + 63) / 64) * p.q_chunk;
+  if (per_photon > 0 && scratch) {
+    const size_t rec = sizeof(R) * 4 * kRecQ + sizeof(double) + sizeof(int32_t) * kRecI;
+    const int64_t cap = static_cast<int64_t>(kScratchBytes / rec) & ~int64_t(1);
+    if (cap - reserve < per_photon)
+      return fail(LMC_ERR_INVALID,
+                  "no-alloc run: the crossing queue needs " + std::to_string((reserve + per_photon) * rec) +
+                      " bytes for one photon per launch (records per photon " + std::to_string(per_photon) +
+                      "), the static scratch holds " + std::to_string(kScratchBytes) + ": use lmc_scene_run");
+    chunk = std::max<int64_t>(1, (cap - reserve) / per_photon);
+    p.qf = reinterpret_cast<R*>(scratch);
+    p.qd = reinterpret_cast<double*>(scratch + sizeof(R) * 4 * kRecQ * cap);
+    p.qi = reinterpret_cast<int32_t*>(scratch + (sizeof(R) * 4 * kRecQ + sizeof(double)) * cap);
+    p.q_cap = cap;
+  } else if (per_photon > 0) {
     const int64_t want = static_cast<int64_t>(std::min<uint64_t>(total, uint64_t(1) << 40)) * per_photon;
     const int64_t cap = std::min<int64_t>(kQueueRecords, want) + reserve;
     int rc = ensure_queue<R>(s, cap);
@@ -859,24 +892,29 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
 }
 
 template <bool kAdj>
-int run_dispatch(lmc_scene* s, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph, lmc_stats* st) {
+int run_dispatch(lmc_scene* s, const lmc_run_desc* r, const uint64_t* budgets, bool use_graph, lmc_stats* st,
+                 unsigned char* scratch = nullptr) {
   if (!s || !r) return fail(LMC_ERR_INVALID, "null argument");
   if (r->max_events < 0) return fail(LMC_ERR_INVALID, "max_events must be >= 0");
   if (!budgets && r->n_photons == 0) {
     if (st) std::memset(st, 0, sizeof(*st));
     return LMC_OK;
   }
+  const char* no_tables = "no-alloc run: this precision's tables are not on the device yet (FP32 tables are "
+                          "uploaded by lmc_scene_create, FP64 ones by a first lmc_scene_run)";
   if (r->precision == LMC_PREC_FP32) {
     if (r->rng != LMC_RNG_PHILOX) return fail(LMC_ERR_UNSUPPORTED, "FP32 transport supports the Philox stream only");
+    if (scratch && !s->t32.ready) return fail(LMC_ERR_INVALID, no_tables);
     int rc = build_tables<float>(s, s->t32);
     if (rc) return rc;
-    return run_impl<float, false, kAdj>(s, s->t32, r, budgets, use_graph, st);
+    return run_impl<float, false, kAdj>(s, s->t32, r, budgets, use_graph, st, scratch);
   }
   if (r->precision == LMC_PREC_FP64) {
+    if (scratch && !s->t64.ready) return fail(LMC_ERR_INVALID, no_tables);
     int rc = build_tables<double>(s, s->t64);
     if (rc) return rc;
-    if (r->rng == LMC_RNG_XOSHIRO) return run_impl<double, true, kAdj>(s, s->t64, r, budgets, use_graph, st);
-    return run_impl<double, false, kAdj>(s, s->t64, r, budgets, use_graph, st);
+    if (r->rng == LMC_RNG_XOSHIRO) return run_impl<double, true, kAdj>(s, s->t64, r, budgets, use_graph, st, scratch);
+    return run_impl<double, false, kAdj>(s, s->t64, r, budgets, use_graph, st, scratch);
   }
   return fail(LMC_ERR_INVALID, "unknown precision");
 }
@@ -1213,6 +1251,8 @@ int lmc_scene_create(const lmc_scene_desc* d, lmc_scene** out) {
   rc = add_scene(s, d, true);
   if (!rc) rc = finish_set(s);
   if (!rc) rc = zero_set(s);
+  // the production tables now, so a no-alloc run can follow at once
+  if (!rc) rc = build_tables<float>(s, s->t32);
   if (rc) {
     destroy_set(s);
     delete s;
@@ -1235,6 +1275,32 @@ int lmc_scene_zero(lmc_scene* s) {
 
 int lmc_scene_run(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) {
   return run_dispatch<false>(s, r, nullptr, false, st);
+}
+
+// lmc_scene_run without any device allocation: the crossing queue is the
+// static `scratch` array.  Runs on one device are serialised by a mutex
+// (they share that device's array); the array keeps the last run's records,
+// and each launch rewrites the plane sentinels it can hand out before use.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int lmc_scene_run_noalloc(lmc_scene* s, const lmc_run_desc* r, lmc_stats* st) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  int dev = 0;
+  LMC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LMC_ERR_DEVICE, "device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(LMC_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return run_dispatch<false>(s, r, nullptr, false, st, g_scratch_ptr[dev]);
 }
 
 int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits) {

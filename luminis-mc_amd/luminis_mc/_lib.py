@@ -159,6 +159,7 @@ def lib():
     L.lmc_scene_destroy.argtypes = [ctypes.c_void_p]
     L.lmc_scene_destroy.restype = None
     L.lmc_scene_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunDesc), ctypes.POINTER(Stats)]
+    L.lmc_scene_run_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunDesc), ctypes.POINTER(Stats)]
     L.lmc_scene_read.argtypes = [ctypes.c_void_p, c_dp, c_int64, ctypes.POINTER(c_uint64)]
     L.lmc_scene_zero.argtypes = [ctypes.c_void_p]
     L.lmc_scene_records.argtypes = [ctypes.c_void_p, c_dp, c_int64, ctypes.POINTER(c_int64)]
