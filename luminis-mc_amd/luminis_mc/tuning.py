@@ -20,9 +20,12 @@ from . import _lib
 
 # (block_size, blocks_per_cu, claim): 16 resident waves per CU is the target
 # occupancy at <= 128 VGPRs; the candidates vary how they are grouped and how
-# often a wave returns to the global index counter.
+# often a wave returns to the global index counter.  Claims of 64 (the
+# library's single-scene default) and 8 (its estimator default) keep short
+# runs' tails short; 256 keeps a batch's waves inside one scene.
 DEFAULT_CANDIDATES = (
     (256, 4, 256), (512, 2, 256), (1024, 1, 256), (256, 4, 128), (256, 4, 512), (128, 8, 256), (256, 3, 256),
+    (512, 2, 64), (1024, 1, 64), (256, 4, 64), (256, 4, 8),
 )
 
 _memo: dict = {}

@@ -348,3 +348,46 @@ def test_photon_and_source_distributions():
     assert abs(r2m - 0.5) < 5 * math.sqrt(1 / 12 / n) and (d[:, 0] ** 2 + d[:, 1] ** 2).max() <= 1.0
     g = np.array([lmc.gaussian_distribution(rng, V(), 2.0).as_tuple() for _ in range(n)])
     assert abs(g[:, 0].var() - 4.0) < 5 * 4.0 * math.sqrt(2 / n) and abs(g[:, 1].mean()) < 5 * 2.0 / math.sqrt(n)
+
+
+def test_autotuner_picks_the_fastest_usable_shape(monkeypatch, tmp_path):
+    """The tuner times every candidate on a scaled-down copy of the run (the
+    same scenes at `fraction` of their budgets, every non-empty scene at least
+    one photon), skips shapes the run refuses, zeroes the target, returns the
+    fastest and caches it in memory and on disk."""
+    from luminis_mc import tuning
+    monkeypatch.setattr(tuning, "_device_name", lambda: "gfx950:256")
+    monkeypatch.setattr(tuning, "_memo", {})
+    cost = {(256, 4, 256): 9.0, (512, 2, 256): 7.5, (1024, 1, 256): 6.0, (1024, 1, 64): 5.5, (256, 4, 8): 8.0}
+
+    class FakeBatch:
+        def __init__(self):
+            self.calls, self.zeroed = [], 0
+
+        def __len__(self):
+            return 3
+
+        def run(self, budgets, r, use_graph=False):
+            shape = (r.block_size, r.blocks_per_cu, r.claim)
+            self.calls.append((shape, [int(b) for b in budgets]))
+            if shape == (1024, 1, 64):
+                raise NotImplementedError("refused shape")
+            return {"kernel_ms": cost.get(shape, 10.0)}
+
+        def zero(self):
+            self.zeroed += 1
+
+    t = FakeBatch()
+    budgets = np.array([10_000_000, 0, 3], dtype=np.uint64)
+    cache = str(tmp_path / "tune.json")
+    res = tuning.autotune(t, _lib.RunDesc(), budgets=budgets, fraction=0.05, min_photons=1000, cache_path=cache)
+    assert (res["block_size"], res["blocks_per_cu"], res["claim"]) == (1024, 1, 256)
+    assert len(res["table"]) == len(tuning.DEFAULT_CANDIDATES) - 1  # the refused shape is skipped
+    assert {c for c, _ in t.calls} == set(tuning.DEFAULT_CANDIDATES)
+    assert all(b == [500_000, 0, 1] for _, b in t.calls) and t.zeroed == 1
+    assert (512, 2, 64) in tuning.DEFAULT_CANDIDATES and (256, 4, 8) in tuning.DEFAULT_CANDIDATES
+    n_calls = len(t.calls)
+    assert tuning.autotune(t, _lib.RunDesc(), budgets=budgets, cache_path=cache)["ms"] == 6.0 and len(t.calls) == n_calls
+    monkeypatch.setattr(tuning, "_memo", {})
+    assert tuning.autotune(t, _lib.RunDesc(), budgets=budgets, cache_path=cache)["claim"] == 256  # from disk
+    assert len(t.calls) == n_calls
