@@ -724,15 +724,26 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
 // starts on the layer interface the photon has just crossed (FP32 runs place
 // it exactly there; the reference, and the FP64 build, nudge it 1e-12 past),
 // so a plane at that interface, already recorded by the sub-step that ended
-// there, is not recorded again.
-template <class R, bool kXo, bool kAdj, int kAux>
+// there, is not recorded again.  A single scene's planes are the same for
+// every lane: the loop runs over them with a wave-uniform index (kernel
+// argument bound, scalar loop), so the common no-crossing case costs two
+// compares per plane.  A batch's lanes can sit in different scenes and take
+// their plane range from their layer.
+template <class R, bool kXo, bool kAdj, int kAux, bool kBatch>
 __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, V3<R> prev,
                                            bool from_iface) {
   const R z1 = prev.z, z2 = ph.pos.z;
   if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
   const R zmin = z1 < z2 ? z1 : z2, zmax = z1 < z2 ? z2 : z1;
   bool killed = false;
-  const int pl0 = v.L[ph.layer].pl_first, pl1 = pl0 + v.L[ph.layer].pl_count;
+  int pl0, pl1;
+  if constexpr (kBatch) {
+    pl0 = v.L[ph.layer].pl_first;
+    pl1 = pl0 + v.L[ph.layer].pl_count;
+  } else {
+    pl0 = 0;
+    pl1 = p.n_planes;
+  }
   for (int pl = pl0; pl < pl1; ++pl) {
     const R zp = static_cast<R>(v.planes[pl].z);
     if (zp >= zmin && zp <= zmax && !(from_iface && zp == z1))
@@ -1074,7 +1085,7 @@ template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path
 
 // One iteration of run_photon's loop (simulation.cpp:297-614).  Returns the
 // number of azimuth trials used.
-template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, int kAux = 1>
+template <class R, bool kXo, bool kTrack, bool kAdj, int kPart, int kAux = 1, bool kBatch = false>
 __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph,
                                                bool renorm = true) {
   // kPart 0: the whole event; 1: the flight up to the species of the next
@@ -1100,67 +1111,71 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     }
     R step = -v.L[ph.layer].mfp * M<R>::log(u_fp);
     const V3<R> dir = ph.P.s;
-    R new_z = ph.pos.z + dir.z * step;
     bool on_iface = false;  // FP32: the photon stands exactly on the interface it has just crossed
-    // interface crossings (simulation.cpp:317-383), with the layer index tracked
-    // explicitly (the reference nudges z by 1e-12 and re-derives it)
+    // The flight in segments (simulation.cpp:317-407): a segment ends at the
+    // current layer's face when the remaining step crosses it (the remaining
+    // optical depth then carries into the next layer, :336,373) and otherwise
+    // at the step's end; each segment's sensor-plane crossings are recorded
+    // (one record_hit per segment, a single copy of the plane code).  The layer
+    // index is tracked explicitly (the reference nudges z by 1e-12 past the
+    // interface and re-derives it; the FP64 build nudges too).
     for (;;) {
+      const LayerDev<R>& Lc = v.L[ph.layer];
+      const R new_z = ph.pos.z + dir.z * step;
       R zi = R(0);
       bool has = false;
       if (new_z > ph.pos.z) {
-        if (v.L[ph.layer].nbr & 1) {
-          zi = v.L[ph.layer].z_max;
+        if (Lc.nbr & 1) {
+          zi = Lc.z_max;
           has = zi <= new_z;
         }
       } else if (new_z < ph.pos.z) {
-        if (v.L[ph.layer].nbr & 2) {
-          zi = v.L[ph.layer].z_min;
+        if (Lc.nbr & 2) {
+          zi = Lc.z_min;
           has = zi >= new_z;
         }
       }
-      if (!has) break;
-      const R dzt = dir.z * step;
-      if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) break;
-      const R t = (zi - ph.pos.z) / dzt;
-      if (t < R(0) || t > R(1)) break;
-      const R step_to = step * t;
-      const R rem = (step - step_to) * v.L[ph.layer].mu_t;
-      ph.gpl += step_to;
-      ph.opl += layer_n64(v.L[ph.layer]) * static_cast<double>(step_to);
-      add_path<R, kXo, kAdj>(ph, step_to);
+      R seg = step;
+      if (has) {
+        const R dzt = dir.z * step;
+        if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) {
+          has = false;
+        } else {
+          const R t = (zi - ph.pos.z) / dzt;
+          if (t < R(0) || t > R(1))
+            has = false;
+          else
+            seg = step * t;
+        }
+      }
+      ph.gpl += seg;
+      ph.opl += layer_n64(Lc) * static_cast<double>(seg);
+      add_path<R, kXo, kAdj>(ph, seg);
       const V3<R> prev = ph.pos;
-      ph.pos.x += dir.x * step_to;
-      ph.pos.y += dir.y * step_to;
+      ph.pos.x += dir.x * seg;
+      ph.pos.y += dir.y * seg;
       const bool up = dir.z > R(0);
-      if constexpr (sizeof(R) == 8)
-        ph.pos.z = zi + (up ? 1e-12 : -1e-12);
-      else
-        ph.pos.z = zi;
-      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev, on_iface)) {
+      if (has) {
+        if constexpr (sizeof(R) == 8)
+          ph.pos.z = zi + (up ? 1e-12 : -1e-12);
+        else
+          ph.pos.z = zi;
+      } else {
+        ph.pos.z += dir.z * seg;
+        if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
+          if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
+        }
+      }
+      if (record_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, prev, on_iface)) {
         ph.alive = false;
         return 0;
       }
+      if (!has) break;
+      const R rem = (step - seg) * Lc.mu_t;
       on_iface = sizeof(R) == 4;
       ph.layer += up ? 1 : -1;
       const R mt = v.L[ph.layer].mu_t;
       step = mt > R(0) ? rem / mt : rem;
-      new_z = ph.pos.z + dir.z * step;
-    }
-    ph.gpl += step;
-    ph.opl += layer_n64(v.L[ph.layer]) * static_cast<double>(step);
-    add_path<R, kXo, kAdj>(ph, step);
-    {
-      const V3<R> prev = ph.pos;
-      ph.pos.x += dir.x * step;
-      ph.pos.y += dir.y * step;
-      ph.pos.z += dir.z * step;
-      if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
-        if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
-      }
-      if (record_hit<R, kXo, kAdj, kAux>(p, v, ph, prev, on_iface)) {
-        ph.alive = false;
-        return 0;
-      }
     }
     // Sample::is_inside (sample.cpp:268-288)
     if (!(ph.pos.z >= R(0)) || ph.pos.z >= v.L[ph.layer].z_top || !isfinite(ph.pos.x) ||
@@ -1555,13 +1570,13 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     }
     int32_t ev0 = ph.events;
     if constexpr (kEst) {
-      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAux>(p, v, ph, renorm);
+      if (ph.alive) transport_event<R, kXo, kTrack, kAdj, 1, kAux, kBatch>(p, v, ph, renorm);
       // SensorsGroup::run_estimators (detector.cpp:272-281) with the whole wave
       for (int si = 0; si < p.n_sensors; ++si)
         if (v.sens[si].est && v.sens[si].kind == LMC_SENSOR_FARFIELD_CBS) estimate_wave<R, kXo>(p, v, ph, si);
-      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAux>(p, v, ph, renorm);
+      if (ph.alive) w_tr += transport_event<R, kXo, kTrack, kAdj, 2, kAux, kBatch>(p, v, ph, renorm);
     } else if (ph.alive) {
-      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAux>(p, v, ph, renorm);
+      const int tr = transport_event<R, kXo, kTrack, kAdj, 0, kAux, kBatch>(p, v, ph, renorm);
       w_tr += tr;  // per-lane trial counts summed below
     }
     w_ev += static_cast<uint64_t>(__popcll(__ballot(ph.events != ev0)));

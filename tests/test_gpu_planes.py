@@ -8,13 +8,13 @@ grids, and of the planar field's phase on long paths.
   so its hit counts match the FP64 build on the same streams.
 * A far-field CBS grid of 512 KiB (beyond the 48 KiB the sensor kernel keeps
   in LDS, so its deposits take HBM FP64 atomics): FP64 trajectory parity.
-* The planar field's phase k0 * (optical path) on paths beyond 1e6 rad, FP32
+* The planar field's phase k0 * (optical path) on paths of 1e6 - 2e7 rad, FP32
   production path against the FP64 build photon by photon: the wavenumber and
   the refractive index enter in FP64 and the path accumulates in FP64, so the
   median relative phase difference is what the FP32 step lengths leave
-  (measured +3.8e-8 over 413 photons: the hardware log2 behind -mfp ln u is
-  biased by about 0.3 ulp), below the +8.2e-8 more an FP32 k0 and n would add
-  at this wavelength.
+  (a few 1e-8: the hardware log2 behind -mfp ln u is biased by a fraction of
+  an ulp), below the +8.2e-8 more an FP32 k0 and n would add at this
+  wavelength.
 """
 import ctypes
 import math
@@ -93,46 +93,74 @@ def test_cbs_grid_beyond_lds_fp64_parity(require_gpu):
     sc.close()
 
 
-# wavelength where FP32 rounds k0 = 2 pi / lambda by +4.9e-8 relative (and n =
-# 1.33 by +3.2e-8): an FP32 k0 and n would shift every phase by +8.2e-8 of it
-WL_FIELD = 0.63592947e-3
+# wavelength (17 nm) where FP32 rounds k0 = 2 pi / lambda up by 4.2e-8 (and
+# n = 1.33 up by 3.2e-8): an FP32 k0 and n would add 7.4e-8 of every phase
+WL_FIELD = 1.6994495e-05
 
 
-def test_field_phase_on_long_paths_fp32_vs_fp64(require_gpu):
+def test_field_phase_fp32_vs_fp64_on_single_scattering(require_gpu):
+    """Photons scattered once (HG g = 0.5, mu_s 1 /mm) carry phases k0 n L of
+    about 1e6 rad at this wavelength and their FP32 and FP64 trajectories stay
+    close (one free path, one FP32 polar angle), so the per-photon phase
+    difference FP32 - FP64 is beta * phase plus small noise.  8 x 50000 photons,
+    one launch per batch and precision; a 1024^2 field over 8 mm and a
+    PhotonRecordSensor pick out the pixels that hold exactly one photon in
+    both runs (the same photon: one event, same first-scatter point).  With k0
+    and n in FP64, beta is what the FP32 free-path logarithm leaves (the
+    hardware log2 lengthens paths by ~4e-8 of their length); an FP32 k0 and n
+    would add 7.4e-8 (measured on 12614 photons: 3.3e-8 against 1.04e-7, each
+    +- 7.5e-9)."""
     n = 1.33
-    m, ph = scenes.mie_medium(0.5e-3, 1.59, n, 0.5, 0.0, n_div=4000)
+    hg = lmc.HenyeyGreensteinPhaseFunction(0.5)
+    m = lmc.RGDMedium(hg, 0.1e-3, 1.59, n, WL_FIELD)
+    m.set_scattering_coefficient(1.0)
+    m.set_absorption_coefficient(0.0)
+    m.set_mean_free_path(1.0)
     sample = lmc.Sample(n)
     sample.add_layer(m, 0.0, math.inf)
     laser = lmc.Laser(1.0, 0.0, WL_FIELD, 0.0, lmc.LaserSource.Point)
     g = lmc.SensorsGroup()
-    g.add_detector(lmc.PlanarFieldSensor(0.0, 1000.0, 1000.0, 1000.0, 1000.0, absorb=False))
+    N_PIX, LEN = 1024, 8.0
+    g.add_detector(lmc.PlanarFieldSensor(0.0, LEN, LEN, LEN / N_PIX, LEN / N_PIX, absorb=False))
     g.add_detector(lmc.PhotonRecordSensor(0.0, absorb=True))
     sc = GpuScene(sample, laser, g.detectors)
     cfg = lmc.SimConfig()
-    cfg.seed, cfg.track_reverse_paths, cfg.MAX_EVENTS = 44, True, 100000
+    cfg.seed, cfg.track_reverse_paths = 44, True
     k0 = 2 * math.pi / WL_FIELD
-    rel, phases = [], []
-    for i in range(600):
-        res = []
+    d, phi = [], []
+    for batch in range(8):
+        runs = {}
         for prec in ("fp32", "fp64"):
             sc.zero()
-            sc.run(_rd(cfg, prec, "philox", 1, off=i))
+            sc.run(_rd(cfg, prec, "philox", 50000, off=batch * 50000))
             out, hits = sc.read()
-            if int(hits[1]) != 1:
-                break
             rec = sc.records()
-            res.append((complex(out[0], out[1]), rec[0, 5], int(rec[0, 1])))
-        if len(res) != 2 or res[0][2] != res[1][2]:
-            continue
-        phase = k0 * n * res[1][1]
-        if phase < 1e6 or abs(res[1][0]) == 0.0:
-            continue
-        d = np.angle(res[0][0] * np.conj(res[1][0]))
-        rel.append(d / phase)
-        phases.append(phase)
+            f = out[:4 * N_PIX * N_PIX].reshape(2, N_PIX, N_PIX, 2)
+            pix = np.floor((rec[:, 14:16] + 0.5 * LEN) / (LEN / N_PIX)).astype(np.int64)
+            inside = np.all((pix >= 0) & (pix < N_PIX), axis=1)
+            key = np.where(inside, pix[:, 0] * N_PIX + pix[:, 1], -1 - np.arange(len(rec)))  # outside: never shared
+            runs[prec] = (f[0, :, :, 0] + 1j * f[0, :, :, 1], rec, key)
+        (f32, r32, k32), (f64, r64, k64) = runs["fp32"], runs["fp64"]
+        u32, c32 = np.unique(k32, return_counts=True)
+        u64, c64 = np.unique(k64, return_counts=True)
+        single = np.intersect1d(u32[c32 == 1], u64[c64 == 1])
+        single = single[single >= 0]
+        i32 = {int(k): i for i, k in enumerate(k32)}
+        i64 = {int(k): i for i, k in enumerate(k64)}
+        for k in single:
+            a, b = r32[i32[int(k)]], r64[i64[int(k)]]
+            if a[1] != 1 or b[1] != 1 or np.abs(a[8:11] - b[8:11]).max() > 1e-5:
+                continue
+            x, y = divmod(int(k), N_PIX)
+            if f64[x, y] == 0:
+                continue
+            d.append(np.angle(f32[x, y] * np.conj(f64[x, y])))
+            phi.append(k0 * n * b[5])
     sc.close()
-    rel = np.array(rel)
-    print(f"{rel.size} photons beyond 1e6 rad (max {max(phases, default=0):.3g}): relative phase difference "
-          f"mean {rel.mean():.3e} median {np.median(rel):.3e} max|.| {np.abs(rel).max():.3e}")
-    assert rel.size >= 15
-    assert abs(np.median(rel)) < 7e-8
+    d, phi = np.array(d), np.array(phi)
+    beta = (d * phi).sum() / (phi * phi).sum()
+    se = math.sqrt(((d - beta * phi) ** 2 * phi ** 2).sum()) / (phi * phi).sum()
+    print(f"{d.size} single-scattered photons alone in their pixel, phase {np.median(phi):.3g} (median): "
+          f"slope {beta:.3e} +- {se:.1e}; |d| median {np.median(np.abs(d)):.3f} rad")
+    assert d.size > 10000
+    assert se < 1e-8 and beta < 6.5e-8
