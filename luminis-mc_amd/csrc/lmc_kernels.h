@@ -1216,28 +1216,57 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     uth = static_cast<R>(ph.xo.s.uniform());
   else
     uth = u01<R>(ph.w_th);
-  R theta;
+  R theta, st, ct;
   const int kind = S.kind;
-  if (kind == LMC_PHASE_HG) {  // phase.cpp:108-115
-    const R g = S.hg_g, gg = g * g;
-    const R frac = qdiv(R(1) - gg, R(1) - g + R(2) * g * uth);
-    R c = qdiv(R(1) + gg - frac * frac, R(2) * g);
-    if constexpr (sizeof(R) == 4) c = clamp1(c);
-    theta = M<R>::acos(c);
-    if constexpr (kAdj) {
+  if constexpr (sizeof(R) == 4) {
+    // One divergent branch only (the table search of the table kinds); the
+    // closed forms are selects over the whole wave, so a wave whose lanes sit
+    // in layers of different phase kinds pays no extra branch per kind.
+    // Cosine of the uniform (phase.cpp:78-83) and HG (:108-115) kinds:
+    R c = R(2) * uth - R(1);
+    if (__ballot(kind == LMC_PHASE_HG)) {  // wave-uniform
+      const bool hg = kind == LMC_PHASE_HG;
+      const R g = hg ? S.hg_g : R(0.5), gg = g * g;
+      const R frac = qdiv(R(1) - gg, R(1) - g + R(2) * g * uth);
+      const R ch = qdiv(R(1) + gg - frac * frac, R(2) * g);
+      c = hg ? ch : c;
+      if constexpr (kAdj) {
+        const R cc = clamp1(ch);
 #pragma unroll
-      for (int l = 0; l < kMaxGradLayers; ++l)
-        if (l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - c) / (R(1) + gg - R(2) * g * c);
+        for (int l = 0; l < kMaxGradLayers; ++l)
+          if (hg && l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - cc) / (R(1) + gg - R(2) * g * cc);
+      }
     }
-  } else if (kind == LMC_PHASE_TABLE_THETA) {
-    theta = table_sample(v, S, uth);
-  } else if (kind == LMC_PHASE_TABLE_COS) {
-    theta = M<R>::acos(table_sample(v, S, uth));
+    R t = R(0);
+    if (kind == LMC_PHASE_TABLE_THETA || kind == LMC_PHASE_TABLE_COS) t = table_sample(v, S, uth);
+    const bool direct = kind == LMC_PHASE_TABLE_THETA;
+    c = clamp1(kind == LMC_PHASE_TABLE_COS ? t : c);
+    theta = t;
+    if (__ballot(!direct)) {  // wave-uniform
+      const R tc = M<R>::acos(c);
+      theta = direct ? t : tc;
+    }
+    M<R>::sincos(theta, &st, &ct);
   } else {
-    theta = M<R>::acos(R(2) * uth - R(1));
+    if (kind == LMC_PHASE_HG) {  // phase.cpp:108-115
+      const R g = S.hg_g, gg = g * g;
+      const R frac = qdiv(R(1) - gg, R(1) - g + R(2) * g * uth);
+      R c = qdiv(R(1) + gg - frac * frac, R(2) * g);
+      theta = M<R>::acos(c);
+      if constexpr (kAdj) {
+#pragma unroll
+        for (int l = 0; l < kMaxGradLayers; ++l)
+          if (l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - c) / (R(1) + gg - R(2) * g * c);
+      }
+    } else if (kind == LMC_PHASE_TABLE_THETA) {
+      theta = table_sample(v, S, uth);
+    } else if (kind == LMC_PHASE_TABLE_COS) {
+      theta = M<R>::acos(table_sample(v, S, uth));
+    } else {
+      theta = M<R>::acos(R(2) * uth - R(1));
+    }
+    M<R>::sincos(theta, &st, &ct);
   }
-  R st, ct;
-  M<R>::sincos(theta, &st, &ct);
   cx<R> s1, s2;
   s_lookup(v.stab, S, theta, s1, s2);
   // azimuth by rejection against F(phi) = a cos^2 + b sin^2 + c sin cos (phase.cpp:27-49)
@@ -1398,6 +1427,10 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
 // species, sensors and planes with immediate offsets instead of keeping their
 // addresses in scalar registers.  Staged tables (when they fit) follow `end`.
 constexpr size_t al16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
+// Dead lanes a wave gathers before it refills them: refilling on every death
+// runs the launch code in ~1 of 4 iterations on 240-event paths; waiting for
+// three idles ~1.5 % of the lanes and runs it a third as often.
+constexpr int kRefillMin = 3;
 template <class R> struct LdsHdr {
   static constexpr size_t hits = 0;
   static constexpr size_t wq_base = al16(hits + sizeof(unsigned) * (kMaxSensors + 2));
@@ -1524,8 +1557,10 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   uint32_t iter = 0;
   for (;;) {
     const bool renorm = (++iter & 1u) == 0u;  // wave-uniform: T normalised every other iteration
+    // Refill once a few lanes are dead (the launch code runs for the whole
+    // wave whenever any lane takes a photon): kRefillMin dead lanes, or all.
     const uint64_t dead = __ballot(!ph.alive);
-    if (dead != 0 && !exhausted) {
+    if (!exhausted && (__popcll(dead) >= kRefillMin || dead == ~0ull)) {
       if (pool_next >= pool_end) {
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(p.counter, static_cast<unsigned long long>(p.claim));
