@@ -347,6 +347,11 @@ enum : int32_t { kPlaneCbs = 1, kPlaneRec = 2 };
 struct PlaneDev {
   double z;
   int32_t first, count, flags, scene;  // scene: index of its laser constants
+  float zf;                            // z rounded to FP32: the FP32 kernels' crossing tests
+  int32_t pad;
 };
+template <class R> __device__ __forceinline__ R plane_z(const PlaneDev& P);
+template <> __device__ __forceinline__ float plane_z<float>(const PlaneDev& P) { return P.zf; }
+template <> __device__ __forceinline__ double plane_z<double>(const PlaneDev& P) { return P.z; }
 
 }  // namespace lmc

@@ -70,6 +70,7 @@ template <class R> struct KParams {
   int32_t abs_on, abs_nr, abs_nz, abs_nt;
   int64_t abs_off;
   int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
+  R pz_lo, pz_hi;                // lowest and highest sensor plane (as R) of all scenes of the launch
   // PhotonRecordSensor records of one sensor-kernel launch (LMC_RECORD_FIELDS
   // doubles each), [0] records written, [1] records that found no room
   double* rec;
@@ -151,6 +152,10 @@ template <class R, bool kXo> struct Photon {
   int32_t cur_sp;    // species of the pending vertex between the two parts of an estimator event
   Xoshiro128ss rng;  // this photon's event stream (seeded from Philox at launch)
   bool alive;
+  // FP32 kernels: 1 when the photon's last flight segment left the sample
+  // with its plane crossings not yet recorded (the segment's start is kept in
+  // w_th, w_ph, w_az), 2 when that segment also started on an interface
+  int32_t pend;
   // CBS history
   V3<R> m1, s1, mq, sq;  // frame after the first scatter, frame before the last one
   V3<R> r1, rn;
@@ -622,9 +627,9 @@ __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R ar
 template <class R, bool kXo, bool kAdj, int kAux>
 __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
                                           V3<R> prev, int cdir) {
-  const R zp = static_cast<R>(v.planes[pl].z);
+  const R zp = plane_z<R>(v.planes[pl]);
   const V3<R> d{ph.pos.x - prev.x, ph.pos.y - prev.y, ph.pos.z - prev.z};
-  const R t = (zp - prev.z) / d.z;
+  const R t = qdiv(zp - prev.z, d.z);
   const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
   const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
   const R corr = M<R>::sqrt(dot(ov, ov));
@@ -735,6 +740,9 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   const R z1 = prev.z, z2 = ph.pos.z;
   if (M<R>::fabs(z2 - z1) < static_cast<R>(1e-12)) return false;
   const R zmin = z1 < z2 ? z1 : z2, zmax = z1 < z2 ? z2 : z1;
+  // no plane of any scene of the launch within the segment, for every lane
+  // of the wave: the common case, one compare pair and a uniform branch
+  if (!__ballot(zmin <= p.pz_hi && zmax >= p.pz_lo)) return false;
   bool killed = false;
   int pl0, pl1;
   if constexpr (kBatch) {
@@ -745,7 +753,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
     pl1 = p.n_planes;
   }
   for (int pl = pl0; pl < pl1; ++pl) {
-    const R zp = static_cast<R>(v.planes[pl].z);
+    const R zp = plane_z<R>(v.planes[pl]);
     if (zp >= zmin && zp <= zmax && !(from_iface && zp == z1))
       killed |= plane_hit<R, kXo, kAdj, kAux>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
   }
@@ -1052,6 +1060,7 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
   ph.events = 0;
   ph.layer = L.launch_layer;
   ph.alive = true;
+  ph.pend = 0;
   ph.m1 = ph.P.m;
   ph.s1 = ph.P.s;
   ph.mq = ph.P.m;
@@ -1122,32 +1131,16 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     for (;;) {
       const LayerDev<R>& Lc = v.L[ph.layer];
       const R new_z = ph.pos.z + dir.z * step;
-      R zi = R(0);
-      bool has = false;
-      if (new_z > ph.pos.z) {
-        if (Lc.nbr & 1) {
-          zi = Lc.z_max;
-          has = zi <= new_z;
-        }
-      } else if (new_z < ph.pos.z) {
-        if (Lc.nbr & 2) {
-          zi = Lc.z_min;
-          has = zi >= new_z;
-        }
-      }
-      R seg = step;
-      if (has) {
-        const R dzt = dir.z * step;
-        if (M<R>::fabs(dzt) < static_cast<R>(1e-15)) {
-          has = false;
-        } else {
-          const R t = (zi - ph.pos.z) / dzt;
-          if (t < R(0) || t > R(1))
-            has = false;
-          else
-            seg = step * t;
-        }
-      }
+      // the face the step heads for (z_max going deeper, z_min going back)
+      // and whether it is crossed, as selects: no branch per lane
+      const bool fwd = new_z > ph.pos.z, bwd = new_z < ph.pos.z;
+      const R zi = fwd ? Lc.z_max : Lc.z_min;
+      const int nb = Lc.nbr;
+      const R dzt = dir.z * step;
+      const R t = qdiv(zi - ph.pos.z, dzt);
+      const bool has = ((fwd && (nb & 1) && zi <= new_z) || (bwd && (nb & 2) && zi >= new_z)) &&
+                       !(M<R>::fabs(dzt) < static_cast<R>(1e-15)) && !(t < R(0) || t > R(1));
+      const R seg = has ? step * t : step;
       ph.gpl += seg;
       ph.opl += layer_n64(Lc) * static_cast<double>(seg);
       add_path<R, kXo, kAdj>(ph, seg);
@@ -1155,15 +1148,25 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       ph.pos.x += dir.x * seg;
       ph.pos.y += dir.y * seg;
       const bool up = dir.z > R(0);
-      if (has) {
-        if constexpr (sizeof(R) == 8)
-          ph.pos.z = zi + (up ? 1e-12 : -1e-12);
-        else
-          ph.pos.z = zi;
-      } else {
-        ph.pos.z += dir.z * seg;
-        if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
-          if (ph.pos.z > ph.depth) ph.depth = ph.pos.z;
+      R z_iface = zi;
+      if constexpr (sizeof(R) == 8) z_iface = zi + (up ? 1e-12 : -1e-12);
+      ph.pos.z = has ? z_iface : ph.pos.z + dir.z * seg;
+      if constexpr (kAux >= 1) {  // else 0 in every record: no StatisticsSensor reads it
+        ph.depth = (!has && ph.pos.z > ph.depth) ? ph.pos.z : ph.depth;
+      }
+      if constexpr (sizeof(R) == 4) {
+        // Leaving the sample (Sample::is_inside, sample.cpp:268-288): the
+        // photon dies here whatever its crossings do, so they are recorded
+        // later, together with the other lanes' (a wave runs the plane code
+        // once for all its exits instead of once per exit).
+        if (!has && (!(ph.pos.z >= R(0)) || ph.pos.z >= Lc.z_top || !isfinite(ph.pos.x) || !isfinite(ph.pos.y) ||
+                     !isfinite(ph.pos.z))) {
+          ph.w_th = __float_as_uint(prev.x);
+          ph.w_ph = __float_as_uint(prev.y);
+          ph.w_az = __float_as_uint(prev.z);
+          ph.pend = on_iface ? 2 : 1;
+          ph.alive = false;
+          return 0;
         }
       }
       if (record_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, prev, on_iface)) {
@@ -1549,6 +1552,7 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   // persistent photon loop with wave-level refill from a claimed index pool
   Photon<R, kXo> ph;
   ph.alive = false;
+  ph.pend = 0;
   uint64_t w_ph = 0, w_ev = 0, w_tr = 0;  // wave-uniform counters
   uint64_t pool_next = 0, pool_end = 0;
   bool exhausted = false;
@@ -1560,7 +1564,19 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     // Refill once a few lanes are dead (the launch code runs for the whole
     // wave whenever any lane takes a photon): kRefillMin dead lanes, or all.
     const uint64_t dead = __ballot(!ph.alive);
-    if (!exhausted && (__popcll(dead) >= kRefillMin || dead == ~0ull)) {
+    const bool refill = !exhausted && (__popcll(dead) >= kRefillMin || dead == ~0ull);
+    if constexpr (sizeof(R) == 4) {
+      // record the crossings of the photons that left the sample (their
+      // state is frozen) before their lanes are reused, or while draining
+      if ((refill || exhausted) && __ballot(ph.pend != 0)) {
+        if (ph.pend != 0) {
+          const V3<R> prev{__uint_as_float(ph.w_th), __uint_as_float(ph.w_ph), __uint_as_float(ph.w_az)};
+          record_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, prev, ph.pend == 2);
+          ph.pend = 0;
+        }
+      }
+    }
+    if (refill) {
       if (pool_next >= pool_end) {
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(p.counter, static_cast<unsigned long long>(p.claim));

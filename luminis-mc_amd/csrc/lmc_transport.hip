@@ -611,6 +611,12 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.n_species = static_cast<int32_t>(s->species.size());
   p.n_sensors = static_cast<int32_t>(s->sensors_host.size());
   p.n_planes = static_cast<int32_t>(s->planes_host.size());
+  p.pz_lo = std::numeric_limits<R>::infinity();
+  p.pz_hi = -std::numeric_limits<R>::infinity();
+  for (const PlaneDev& P : s->planes_host) {
+    p.pz_lo = std::min(p.pz_lo, static_cast<R>(P.z));
+    p.pz_hi = std::max(p.pz_hi, static_cast<R>(P.z));
+  }
   p.n_scenes = n_sc;
   p.tab_bytes = static_cast<int32_t>(T.tab_bytes);
   p.lds_grid = s->lds_grid;
@@ -1021,6 +1027,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
     PlaneDev P;
     std::memset(&P, 0, sizeof(P));
     P.z = zs[pl];
+    P.zf = static_cast<float>(zs[pl]);
     P.first = static_cast<int32_t>(s->sensors_host.size());
     P.count = 0;
     P.scene = scene_idx;
