@@ -253,7 +253,11 @@ typedef struct lmc_stats {
   int32_t block_size;      /* launch shape the run used                      */
   int32_t blocks_per_cu;
   int32_t lds_tables;      /* 1: phase/amplitude tables staged in LDS         */
-  int32_t reserved0;
+  int32_t aux_mode;        /* transport kernel variant: 0 = FP32 forward run
+                              whose sensor planes lie on or outside the
+                              stack's faces and off the launch point
+                              (crossings recorded at exit only, no depth),
+                              1 = general, 2 = with the Absorption grid      */
 } lmc_stats;
 
 /* Device facts used for launch shaping and roofline reporting. */

@@ -38,7 +38,7 @@ template <> struct M<float> {
     p = __builtin_fmaf(p, ax, 0.0889789874f);
     p = __builtin_fmaf(p, ax, -0.2145988016f);
     p = __builtin_fmaf(p, ax, 1.5707963050f);
-    const float r = __fsqrt_rn(1.0f - ax) * p;
+    const float r = __builtin_amdgcn_sqrtf(1.0f - ax) * p;
     return x < 0.0f ? 3.14159265358979f - r : r;
   }
   static __device__ __forceinline__ float atan2(float y, float x) { return atan2f(y, x); }

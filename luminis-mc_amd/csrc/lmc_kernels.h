@@ -235,12 +235,11 @@ __device__ __forceinline__ void azimuth_sample(R a, R b, R c, uint32_t wv, uint3
   // principal eigenvector: (A + Rm, B) for A >= 0, else (B, Rm - A); its
   // squared norm is 2 Rm (Rm + |A|)
   const R t = Rm + M<R>::fabs(A);
-  R c0 = R(1), s0 = R(0);
-  if (Rm > R(0)) {
-    const R nrm = M<R>::rsqrt(R(2) * Rm * t);
-    c0 = (A >= R(0) ? t : B) * nrm;
-    s0 = (A >= R(0) ? B : t) * nrm;
-  }
+  // (1, 0) for an isotropic form (Rm = 0); selects, not a branch
+  const bool ani = Rm > R(0);
+  const R nrm0 = M<R>::rsqrt(ani ? R(2) * Rm * t : R(1));
+  const R c0 = ani ? (A >= R(0) ? t : B) * nrm0 : R(1);
+  const R s0 = ani ? (A >= R(0) ? B : t) * nrm0 : R(0);
   const R v = u01<R>(wv);
   const bool lobe = v * Mm < Rm;
   const R r = lobe ? M<R>::sqrt(qdiv(v * Mm, Rm)) : R(1);
@@ -289,6 +288,19 @@ __device__ __forceinline__ void s_lookup(const STab<R>* stab, const SpeciesDev<R
   const STab<R>* tb = stab + s.s_off;
   const R pos = th * s.inv_ds;
   const int last = s.s_n - 1;
+  if constexpr (sizeof(R) == 4) {
+    // FP32: one clamped interpolation (s_n >= 2) instead of three branches;
+    // pos <= 0 and NaN give node 0, pos >= last gives node last (t = 1)
+    const R pc = fminf(pos > R(0) ? pos : R(0), static_cast<R>(last));
+    const int i = min(static_cast<int>(pc), last - 1);
+    const R t = pc - static_cast<R>(i);
+    cx<R> a1, a2, b1, b2;
+    stab_get(tb[i], a1, a2);
+    stab_get(tb[i + 1], b1, b2);
+    s1 = {a1.r + t * (b1.r - a1.r), a1.i + t * (b1.i - a1.i)};
+    s2 = {a2.r + t * (b2.r - a2.r), a2.i + t * (b2.i - a2.i)};
+    return;
+  }
   if (!(pos > R(0))) {  // also maps a NaN angle to node 0: indices stay in bounds
     stab_get(tb[0], s1, s2);
     return;
@@ -573,7 +585,11 @@ __device__ __forceinline__ bool accepts(const SensorDev& sd, V3<R> hp, V3<R> dir
     if (ct < static_cast<R>(sd.cos_th_min) || ct > static_cast<R>(sd.cos_th_max)) return false;
   }
   if (sd.f_ph_en) {
-    const R phi = exit_phi(dir);
+    // the direction is invariant over a flight; without the opaque copy the
+    // compiler evaluates this atan2 ahead of every flight, hit or not
+    V3<R> d = dir;
+    asm volatile("" : "+v"(d.x), "+v"(d.y));
+    const R phi = exit_phi(d);
     if (phi < static_cast<R>(sd.ph_min) || phi > static_cast<R>(sd.ph_max)) return false;
   }
   if (sd.f_pos_en) {
@@ -620,19 +636,14 @@ __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R ar
   return kind == LMC_SENSOR_REFLECTANCE;
 }
 
-// A crossing of plane `pl` by the segment prev -> pos: hit point and the
-// overshoot-corrected paths (detector.cpp:194-235); the sensors' filters
-// decide acceptance and absorption here, the deposits happen in
-// k_crossings.  Returns true when an absorbing sensor accepted the photon.
+// A crossing of plane `pl` at hit point hp, corr (the overshoot from hp to
+// the segment's end) back along the flight: the overshoot-corrected paths
+// (detector.cpp:194-235); the sensors' filters decide acceptance and
+// absorption here, the deposits happen in k_crossings.  Returns true when an
+// absorbing sensor accepted the photon.
 template <class R, bool kXo, bool kAdj, int kAux>
-__device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
-                                          V3<R> prev, int cdir) {
-  const R zp = plane_z<R>(v.planes[pl]);
-  const V3<R> d{ph.pos.x - prev.x, ph.pos.y - prev.y, ph.pos.z - prev.z};
-  const R t = qdiv(zp - prev.z, d.z);
-  const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
-  const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
-  const R corr = M<R>::sqrt(dot(ov, ov));
+__device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
+                                             V3<R> hp, R corr, int cdir) {
   {
     const uint64_t act = __ballot(1);
     if (__lane_id() == __ffsll(static_cast<unsigned long long>(act)) - 1)
@@ -722,6 +733,19 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   iq[I_SP * c] = ph.sp;
   if constexpr (kAux >= 1) iq[I_DEPTH * c] = __float_as_int(static_cast<float>(ph.depth));
   return killed;
+}
+
+// A crossing of plane `pl` by the segment prev -> pos: hit point by linear
+// interpolation in z (detector.cpp:194-212), then plane_record.
+template <class R, bool kXo, bool kAdj, int kAux>
+__device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
+                                          V3<R> prev, int cdir) {
+  const R zp = plane_z<R>(v.planes[pl]);
+  const V3<R> d{ph.pos.x - prev.x, ph.pos.y - prev.y, ph.pos.z - prev.z};
+  const R t = qdiv(zp - prev.z, d.z);
+  const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
+  const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
+  return plane_record<R, kXo, kAdj, kAux>(p, v, ph, pl, hp, M<R>::sqrt(dot(ov, ov)), cdir);
 }
 
 // SensorsGroup::record_hit (detector.cpp:178-270) for the segment prev->pos:
@@ -1169,9 +1193,11 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
           return 0;
         }
       }
-      if (record_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, prev, on_iface)) {
-        ph.alive = false;
-        return 0;
+      if constexpr (kAux != 0) {  // kAux == 0: no plane a flight inside the sample can cross
+        if (record_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, prev, on_iface)) {
+          ph.alive = false;
+          return 0;
+        }
       }
       if (!has) break;
       const R rem = (step - seg) * Lc.mu_t;
@@ -1405,7 +1431,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       ur = aux_uniform<R>(ph);
     }
     if (ur < static_cast<R>(0.1))
-      ph.w /= static_cast<R>(0.1);
+      ph.w = sizeof(R) == 4 ? ph.w * R(10) : ph.w / static_cast<R>(0.1);  // FP32: w * 10 is w / 0.1 to 1e-17
     else
       ph.alive = false;
   }
@@ -1662,7 +1688,10 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
 // kAux: per-photon outputs beyond the sensors' crossings.  1 tracks the
 // penetration depth (maximum z) that StatisticsSensor histograms; 2 also
 // records the Absorption grid at every vertex; 0 (FP32 Philox forward runs
-// without either) carries no trace of them.
+// without either, whose sensor planes all lie on or outside the stack's faces
+// and off the launch point) carries no trace of them and records plane
+// crossings only when a photon leaves the sample (no plane code in the
+// flight loop).
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, int kAux = 1>
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2)))

@@ -153,6 +153,9 @@ struct lmc_scene {
   std::vector<double> aux_host;
   int est_lds_n = 0;
   bool has_est = false;
+  // a sensor plane that a flight inside the sample can cross: strictly inside
+  // some scene's stack, or through its launch point
+  bool flight_planes = false;
   // PhotonRecordSensor records: a device list per sensor-kernel launch,
   // gathered on the host; rec_per_crossing = most record sensors on one plane
   int rec_per_crossing = 0;
@@ -690,8 +693,11 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   const bool track = r->track_reverse_paths != 0;
   bool has_stats = false;
   for (const SensorDev& sd : s->sensors_host) has_stats = has_stats || sd.kind == LMC_SENSOR_STATISTICS;
-  // kernels without the penetration depth only where nothing reads it
-  const int aux_mode = abs_run ? 2 : ((has_stats || rec_run || est || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
+  // kernels without the penetration depth only where nothing reads it, and
+  // without plane code in the flight loop only where a flight crosses a plane
+  // only when it leaves the sample
+  const int aux_mode =
+      abs_run ? 2 : ((has_stats || rec_run || est || s->flight_planes || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
   auto pick = [&](bool wide, bool lds_t) -> TransportFn<R> {
     return transport_kernel<R, kXo, kAdj>(aux_mode, est, wide, lds_t, batch, track);
   };
@@ -893,6 +899,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
     st->block_size = block;
     st->blocks_per_cu = bpc;
     st->lds_tables = lds_tables ? 1 : 0;
+    st->aux_mode = aux_mode;
   }
   return LMC_OK;
 }
@@ -1128,6 +1135,7 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
     // leaving the sample; an interior plane whose sensors all absorb records
     // at most one crossing as well
     const bool boundary = zs[pl] <= 0.0 || (!std::isinf(ztop_s) && zs[pl] >= ztop_s);
+    if (!boundary || zs[pl] == d->laser.pos[2]) s->flight_planes = true;
     if (boundary || all_absorb)
       sc.n_single_planes++;
     else

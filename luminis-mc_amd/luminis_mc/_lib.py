@@ -124,7 +124,7 @@ class Stats(ctypes.Structure):
         ("crossings", c_uint64), ("cbs_detections", c_uint64),
         ("kernel_ms", c_double), ("transport_ms", c_double), ("sensor_ms", c_double),
         ("transport_launches", c_uint64),
-        ("block_size", c_int32), ("blocks_per_cu", c_int32), ("lds_tables", c_int32), ("reserved0", c_int32),
+        ("block_size", c_int32), ("blocks_per_cu", c_int32), ("lds_tables", c_int32), ("aux_mode", c_int32),
     ]
 
     def as_dict(self):

@@ -65,17 +65,64 @@ def test_interface_plane_fp64_parity(require_gpu):
 def test_interface_plane_fp32_counts_each_crossing_once(require_gpu):
     cfg = _interface_cfg(200000, 42)
     sc = GpuScene(cfg.sample, cfg.laser, cfg.detector.detectors)
-    sc.run(_rd(cfg, "fp32", "philox", 200000))
+    st32 = sc.run(_rd(cfg, "fp32", "philox", 200000))
     a32, h32 = sc.read()
     sc.zero()
     sc.run(_rd(cfg, "fp64", "philox", 200000))
     a64, h64 = sc.read()
     sc.close()
     print("hits fp32", list(h32), "fp64", list(h64))
+    assert st32["aux_mode"] == 1  # a plane inside the sample: in-flight recording
     for k in (4, 5):  # the sensors on the interface
         assert abs(int(h32[k]) - int(h64[k])) <= 0.003 * int(h64[k]) + 10, k
     off, n = sc.build.offsets[5]
     assert abs(a32[off] - a64[off]) <= 0.003 * a64[off]
+
+
+def test_surface_planes_launch_and_exit_crossings(require_gpu):
+    """With every sensor plane on the surface (z = 0) and the laser at
+    z = 1e-9 (laser.cpp:29-40) a flight crosses a plane only when it leaves
+    the sample: the FP32 forward run takes the kernel with no plane code in
+    its flight loop (aux_mode 0) and records exits at the wave's flush, and
+    its hit counts, first time bin and crossing count equal the FP64 build's
+    (in-flight recording) on the same streams.  A laser moved onto the plane
+    (z = 0) makes the first segment cross it (the reference records that
+    crossing, t = 0; here the absorbing CBS sensor, which has no direction
+    filter, takes every photon at launch): such a scene keeps the in-flight
+    recording kernel (aux_mode 1) in FP32 too."""
+    N = 200000
+    from luminis_mc.simulation import SceneBuild
+    for z0 in (None, 0.0):
+        cfg, _ = scenes.config_multilayer(N, seed=45, small=True)
+        b = SceneBuild(cfg.sample, cfg.laser, cfg.detector.detectors)
+        if z0 is not None:
+            b.desc.laser.pos[2] = z0
+        sc = GpuScene.from_build(b)
+        st32 = sc.run(_rd(cfg, "fp32", "philox", N))
+        a32, h32 = sc.read()
+        sc.zero()
+        st64 = sc.run(_rd(cfg, "fp64", "philox", N))
+        a64, h64 = sc.read()
+        sc.close()
+        print("laser z", b.desc.laser.pos[2], "aux", st32["aux_mode"], st64["aux_mode"], "hits fp32", list(h32),
+              "fp64", list(h64), "crossings", st32["crossings"], st64["crossings"])
+        assert st64["aux_mode"] == 1
+        if z0 is None:
+            assert st32["aux_mode"] == 0
+            assert 0 < int(h64[1]) < N  # exits only
+        else:
+            # every photon absorbed at launch (the fluence sensor's time bin of
+            # a launch, path length ~ +-1e-17, is a rounding artifact)
+            assert st32["aux_mode"] == 1
+            for st, h in ((st32, h32), (st64, h64)):
+                assert int(st["crossings"]) == N and int(st["events"]) == 0 and int(h[2]) == int(h[3]) == 0
+            continue
+        for k in range(4):
+            assert abs(int(h32[k]) - int(h64[k])) <= 0.003 * int(h64[k]) + 10, k
+        off, n = b.offsets[1]
+        g32, g64 = a32[off:off + n].reshape(4, 9, 16, 16), a64[off:off + n].reshape(4, 9, 16, 16)
+        assert abs(g32[0, 1].sum() - g64[0, 1].sum()) <= 0.003 * g64[0, 1].sum()
+        assert abs(int(st32["crossings"]) - int(st64["crossings"])) <= 0.003 * int(st64["crossings"])
 
 
 def test_cbs_grid_beyond_lds_fp64_parity(require_gpu):
