@@ -49,8 +49,8 @@ PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no pac
 # counters read for a device-to-device copy of known size (`bench.py
 # --copy-probe 256`): fetch x2.00, write x1.00.  Table reads are cache
 # resident; the writes are the crossing records handed to the sensor kernel.
-HBM_FETCH_B_PER_PHOTON = 705.4 * 1024 * 2.00 / 4e6
-HBM_WRITE_B_PER_PHOTON = 6.381e5 * 1024 * 1.00 / 4e6
+HBM_FETCH_B_PER_PHOTON = 691.8 * 1024 * 2.00 / 4e6
+HBM_WRITE_B_PER_PHOTON = 6.519e5 * 1024 * 1.00 / 4e6
 
 
 def log(msg: str) -> None:
@@ -255,7 +255,8 @@ def batched_leg(frac_plain: float = 0.25, cpu_seconds: float = 0.0, sigma_every:
     pb = {k: ([], []) for k in idx}
     b.zero()
     for j in range(SIGMA_BATCHES):
-        rj = run_desc_from(c, n_photons=0, photon_offset=10 ** 12 + j * int(budgets.max()))
+        # batched runs index a scene's photons below 2^32
+        rj = run_desc_from(c, n_photons=0, photon_offset=10 ** 9 + j * int(budgets.max()))
         b.run(per, rj)
         for k in idx:
             x0, xc = probe(b.read(k)[0])
@@ -277,7 +278,7 @@ def batched_leg(frac_plain: float = 0.25, cpu_seconds: float = 0.0, sigma_every:
     pp = {}
     for k in idx:  # the plain form on the same photon streams as the batched σ runs
         pp[k] = gpu_sigma(scs[k], lambda n_, o_: run_desc_from(c2, n_photons=n_, photon_offset=o_),
-                          int(per[k]) * SIGMA_BATCHES, probe, offset=10 ** 12, stride=int(budgets.max()))
+                          int(per[k]) * SIGMA_BATCHES, probe, offset=10 ** 9, stride=int(budgets.max()))
     for sc in scs:
         sc.close()
 
