@@ -7,6 +7,7 @@ with an explicit error (there is no CPU fallback for the transport path).
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -139,6 +140,31 @@ class DeviceInfo(ctypes.Structure):
 
 
 _lib = None
+HIP_RUNTIME = None  # the HIP runtime library loaded ahead of libluminis_hip.so, if any
+
+
+def _preload_hip_runtime() -> None:
+    """One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64
+    (and HSA runtime) and its libraries ask for it by a name that does not
+    match the system runtime's soname, so a process that loads this library
+    (bound to the system runtime) and torch ends up with two runtimes on one
+    device, and torch then finds no GPU.  When torch is installed, its runtime
+    is loaded here first, without importing torch: the library's dependency on
+    libamdhip64.so.7 then binds to it (same soname), and a later `import
+    torch` reuses it.  LMC_HIP_RUNTIME=system keeps the system runtime."""
+    global HIP_RUNTIME
+    if os.environ.get("LMC_HIP_RUNTIME", "") == "system":
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        HIP_RUNTIME = path
 
 
 def lib():
@@ -150,6 +176,7 @@ def lib():
         raise ImportError(
             f"libluminis_hip.so not found at {LIB_PATH}; build it with "
             "`python luminis-mc_amd/build.py` (the transport path has no CPU fallback)")
+    _preload_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
     L.lmc_abi_version.restype = ctypes.c_int
     L.lmc_last_error.restype = ctypes.c_char_p

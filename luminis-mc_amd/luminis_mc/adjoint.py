@@ -182,18 +182,19 @@ def _transport_fn():
         class TransportFunction(torch.autograd.Function):
             @staticmethod
             def forward(ctx, theta, tr):
+                # outputs and the Jacobian live on theta's device (CPU or GPU)
                 th = theta.detach().to("cpu", torch.float64).numpy()
                 out, jac = tr.evaluate(th, with_grad=theta.requires_grad)
                 if jac is not None:
-                    ctx.save_for_backward(torch.from_numpy(jac))
+                    ctx.save_for_backward(torch.from_numpy(jac).to(theta.device))
                 ctx.theta_meta = (theta.dtype, theta.device)
-                return torch.from_numpy(out)
+                return torch.from_numpy(out).to(theta.device)
 
             @staticmethod
             def backward(ctx, g_out):
                 (jac,) = ctx.saved_tensors
                 dtype, device = ctx.theta_meta
-                g = torch.einsum("lpd,d->lp", jac, g_out.to(torch.float64))
+                g = torch.tensordot(jac, g_out.to(jac.device, torch.float64), dims=([2], [0]))
                 return g.to(device=device, dtype=dtype), None
 
         _FN = TransportFunction

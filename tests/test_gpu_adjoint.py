@@ -9,6 +9,8 @@ The gradients come from the production FP32 transport; the differences from
 the FP64 transport on the same per-photon random streams: a relative step of 1e-3 moves
 1/mu_t and mu_s/mu_t by a few FP32 ulps only, so FP32 rounding of the
 perturbed coefficients would bias the difference quotient by several percent."""
+import os
+
 import numpy as np
 import pytest
 
@@ -64,8 +66,6 @@ def test_adjoint_matches_crn_finite_differences(require_gpu):
 
 
 def test_autograd_backward_applies_the_jacobian(require_gpu):
-    # torch on the CPU only, imported after the library has taken the device:
-    # importing it first would bind the library to torch's bundled HIP runtime
     torch = pytest.importorskip("torch")
     sample, laser, sensors, keep = scenes.scene_adjoint()
     tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=500_000, seed=7)
@@ -87,3 +87,31 @@ def test_autograd_backward_applies_the_jacobian(require_gpu):
     assert np.allclose(theta.grad.numpy(), expect, rtol=1e-3, atol=1e-6 * np.abs(expect).max())
     # the Mie layer's g is not a parameter: its row is zero
     assert theta.grad[0, 2].item() == 0.0
+
+
+def test_autograd_with_device_tensors_in_one_hip_runtime(require_gpu):
+    """theta on the GPU: the library and torch share one HIP runtime (the
+    loader binds the library to torch's), so the outputs, the Jacobian and the
+    gradient stay on the device and equal the CPU-side evaluation."""
+    torch = pytest.importorskip("torch")
+    from luminis_mc import _lib
+    assert torch.cuda.is_available()
+    maps = open("/proc/self/maps").read()
+    assert len({l.split()[-1] for l in maps.splitlines() if "libamdhip64" in l}) == 1
+    if _lib.HIP_RUNTIME:
+        assert os.path.realpath(_lib.HIP_RUNTIME) in maps
+    sample, laser, sensors, keep = scenes.scene_adjoint()
+    tr = lmc.DifferentiableTransport(sample, laser, sensors, n_photons=200_000, seed=8)
+    th0 = tr.theta0()
+    theta = torch.tensor(th0, device="cuda", requires_grad=True)
+    out = tr(theta)
+    assert out.is_cuda and out.dtype == torch.float64 and out.shape == (tr.n_out,)
+    w = torch.zeros(tr.n_out, dtype=torch.float64, device="cuda")
+    w[0] = 1.0
+    w[1:101] = 0.25
+    (out * w).sum().backward()
+    assert theta.grad.is_cuda
+    vals, jac = tr.evaluate(th0, with_grad=True)
+    expect = np.einsum("lpd,d->lp", jac, w.cpu().numpy())
+    assert np.allclose(out.detach().cpu().numpy(), vals, rtol=1e-3, atol=1e-6 * np.abs(vals).max())
+    assert np.allclose(theta.grad.cpu().numpy(), expect, rtol=1e-3, atol=1e-6 * np.abs(expect).max())
