@@ -327,7 +327,8 @@ struct SensorDev {
   float x_min, x_max, y_min, y_max;
   double g[8];
   int64_t out_off;
-  int32_t lds_off, pad2;
+  int32_t lds_off;
+  int32_t bin;  // forward runs: deposits binned by tile and summed in LDS (grids far beyond the caches)
   // estimator mode: est != 0 takes no crossings; a FARFIELD_CBS sensor then
   // deposits every vertex's forced scatter into its first est_imax theta rows
   // (bin geometry at aux in the auxiliary pool), in the transport kernel's

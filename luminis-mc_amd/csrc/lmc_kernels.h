@@ -76,6 +76,23 @@ template <class R> struct KParams {
   double* rec;
   unsigned long long* rec_count;
   int64_t rec_cap;
+  // Binned deposits (forward runs): the sensor kernel runs twice over a range
+  // of records, the first time counting per workgroup the deposits into each
+  // tile of 2^bin_shift accumulators of the grids that live in HBM, the second
+  // time writing them as (accumulator index, value) entries grouped by tile;
+  // k_bin_accumulate then sums each tile in LDS and adds it into `out` once
+  // (see k_bin_accumulate for why: FP64 atomics into HBM are the sensor
+  // kernel's bound).
+  uint32_t* bin_keys;    // entry: accumulator index
+  double* bin_vals;      //        and value
+  uint32_t* bin_pos;     // [tile][workgroup]: deposit counts, then the first entry of each
+  uint32_t* bin_tile;    // first entry of each tile (n_tiles + 1), from the tile totals
+  uint4* bin_items;      // accumulation work items: tile, first entry, entries, 1 if the tile has one item
+  uint32_t* bin_n_items;
+  uint32_t* bin_flag;    // set when the two passes disagree on a workgroup's deposits (never expected)
+  int64_t bin_rec_lo, bin_rec_per_wg;  // records [lo + wg * per_wg, + per_wg) of this pass, per workgroup
+  int64_t bin_cap;       // entries the buffers hold
+  int32_t bin_tiles, bin_shift, bin_wgs, bin_chunk;
 };
 
 // One queued plane crossing accepted by at least one sensor of its plane:
@@ -462,9 +479,32 @@ __device__ __forceinline__ void adj_scores(const KParams<R>& p, const SceneView<
   }
 }
 
-template <class R, bool kAdj>
+// kBin: 0 deposits (LDS or an FP64 atomic in HBM); 1 counts the deposits into
+// grids marked for binning (sd.bin) per tile in the workgroup's LDS counters
+// `lbin` (no side effect otherwise); 2 writes each of those as an entry at its
+// tile's next slot (`lbin`: cursors) and makes the other deposits as with 0.
+// Binned deposits keep zero values, so both passes see the same ones.
+template <class R, bool kAdj, int kBin = 0>
 __device__ __forceinline__ void deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int64_t off,
-                                        double val, const R* score) {
+                                        double val, const R* score, unsigned* lbin = nullptr) {
+  if constexpr (kBin != 0) {
+    static_assert(!kAdj, "binned deposits serve forward runs");
+    if (!sd.bin) {  // LDS copy or FP64 atomic, in the writing pass only
+      if constexpr (kBin == 2) acc_add(lds_grid + sd.lds_off, p.out + sd.out_off, sd.lds != 0, off, val);
+      return;
+    }
+    const uint32_t key = static_cast<uint32_t>(sd.out_off + off);
+    if constexpr (kBin == 1) {
+      atomicAdd(&lbin[key >> p.bin_shift], 1u);
+    } else {
+      const unsigned e = atomicAdd(&lbin[key >> p.bin_shift], 1u);
+      if (e < p.bin_cap) {
+        p.bin_keys[e] = key;
+        p.bin_vals[e] = val;
+      }
+    }
+    return;
+  }
   acc_add(lds_grid + sd.lds_off, p.out + sd.out_off, sd.lds != 0, off, val);
   if constexpr (kAdj) {
     // gradient row q of this value: LDS copy after the value grids when the
@@ -508,7 +548,8 @@ __device__ __forceinline__ void stats_deposit(const KParams<R>& p, const SensorD
   auto temporal = [&](int n, int idx) {
     idx = idx < n - 1 ? idx : n - 1;
     deposit<R, kAdj>(p, sd, lds_grid, base + idx, 1.0, score);
-    if (t_idx >= 1) deposit<R, kAdj>(p, sd, lds_grid, base + static_cast<int64_t>(t_idx) * n + idx, 1.0, score);
+    if (t_idx >= 1)
+      deposit<R, kAdj>(p, sd, lds_grid, base + static_cast<int64_t>(t_idx) * n + idx, 1.0, score);
   };
   if (a[0] != R(0)) {
     const int me = static_cast<int>(a[6]);
@@ -553,16 +594,17 @@ __device__ __forceinline__ void stats_deposit(const KParams<R>& p, const SensorD
 // A time-resolved hit goes to its time bin only; the reference's second copy
 // in the time-integrated bin 0 is the sum of bins 1.. and is formed when the
 // accumulators are read (fold_time_bins), which halves these atomics.
-template <class R, bool kAdj>
+template <class R, bool kAdj, int kBin = 0>
 __device__ __forceinline__ void cbs_deposit(const KParams<R>& p, const SensorDev& sd, double* lds_grid, int64_t bin,
-                                            int t_idx, cx<R> Efx, cx<R> Efy, cx<R> Erx, cx<R> Ery, const R* score) {
+                                            int t_idx, cx<R> Efx, cx<R> Efy, cx<R> Erx, cx<R> Ery, const R* score,
+                                            unsigned* lbin = nullptr) {
   const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
   const int64_t plane = tplane * sd.n2;
   const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
 #pragma unroll 1
   for (int q = 0; q < 8; ++q) {
     const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
-    deposit<R, kAdj>(p, sd, lds_grid, q * plane + t_idx * tplane + bin, static_cast<double>(val), score);
+    deposit<R, kAdj, kBin>(p, sd, lds_grid, q * plane + t_idx * tplane + bin, static_cast<double>(val), score, lbin);
   }
 }
 
@@ -605,9 +647,13 @@ __device__ __forceinline__ bool accepts(const SensorDev& sd, V3<R> hp, V3<R> dir
 // falls outside the grid, where the reference deposits nothing and counts no
 // hit.  t_idx: time bin >= 1 of a time-resolved grid, else 0 (the grid row a
 // deposit goes to; row 0 of a time-resolved grid is completed at read-out).
+// Without FMA contraction: the two binned passes of the sensor kernel must bin
+// every crossing identically, and contraction decided by each inlining
+// context could round a coordinate differently in each.
 template <class R>
 __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R arr, V3<R> dir, int64_t& bin,
                                            int& t_idx) {
+#pragma clang fp contract(off)
   const int kind = sd.kind;
   t_idx = 0;  // time bin; 0 = steady state (the only row when dt == 0)
   bin = 0;
@@ -1735,21 +1781,44 @@ k_transport_est(KParams<R> p) {
 // LDS copy (ds_add_f64), the rest straight to HBM (global_atomic_add_f64).
 // Time-resolved CBS and planar-fluence hits are deposited in their time bin
 // only; bin 0 (time-integrated) is completed at read-out.
-template <class R, bool kAdj>
+// kBin 0: one pass, deposits by FP64 atomics (grids in HBM) or into the
+// workgroup's LDS copy.  kBin 1 and 2: the binned passes over this
+// workgroup's contiguous share of the record range [bin_rec_lo, + wgs *
+// per_wg) (the same records both times): 1 only counts the HBM deposits per
+// tile, 2 does everything else and writes those deposits as entries.
+template <class R, bool kAdj, int kBin = 0>
 __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* sgrid = reinterpret_cast<double*>(smem);
   const int n_gp = 3 * (p.n_layers < kMaxGradLayers ? p.n_layers : kMaxGradLayers);
   const int64_t lds_vals = static_cast<int64_t>(p.lds_grid) * (1 + ((kAdj && p.lds_grad) ? n_gp : 0));
   unsigned* shits = reinterpret_cast<unsigned*>(smem + ((sizeof(double) * lds_vals + 15) & ~static_cast<size_t>(15)));
+  unsigned* lbin = shits + ((p.n_sensors + 1 + 3) & ~3);  // binned passes: per-tile counters / cursors
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int64_t i = tid; i < lds_vals; i += nt) sgrid[i] = 0.0;
+  if constexpr (kBin != 1)
+    for (int64_t i = tid; i < lds_vals; i += nt) sgrid[i] = 0.0;
   for (int i = tid; i < p.n_sensors + 1; i += nt) shits[i] = 0u;
+  if constexpr (kBin == 1)
+    for (int t = tid; t < p.bin_tiles; t += nt) lbin[t] = 0u;
+  if constexpr (kBin == 2)  // first entry of this workgroup's deposits in each tile
+    for (int t = tid; t < p.bin_tiles; t += nt)
+      lbin[t] = p.bin_tile[t] + p.bin_pos[static_cast<int64_t>(t) * p.bin_wgs + blockIdx.x];
   __syncthreads();
   unsigned* s_cbs = shits + p.n_sensors;
   const int64_t n_rec = static_cast<int64_t>(p.q_count[0]) < p.q_cap ? static_cast<int64_t>(p.q_count[0]) : p.q_cap;
   const int64_t c = p.q_cap;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * nt + tid; i < n_rec; i += static_cast<int64_t>(gridDim.x) * nt) {
+  int64_t r_first, r_end, r_step;
+  if constexpr (kBin == 0) {
+    r_first = static_cast<int64_t>(blockIdx.x) * nt + tid;
+    r_end = n_rec;
+    r_step = static_cast<int64_t>(gridDim.x) * nt;
+  } else {
+    const int64_t b0 = p.bin_rec_lo + static_cast<int64_t>(blockIdx.x) * p.bin_rec_per_wg;
+    r_first = b0 + tid;
+    r_end = b0 + p.bin_rec_per_wg < n_rec ? b0 + p.bin_rec_per_wg : n_rec;
+    r_step = nt;
+  }
+  for (int64_t i = r_first; i < r_end; i += r_step) {
     const R* f = p.qf + i * (4 * kRecQ);
     const int32_t* iq = p.qi + i;
     const int pl = iq[I_PLANE * c];
@@ -1789,6 +1858,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
       const int si = first + k;
       const SensorDev& sd = p.sensors[si];
       if (sd.kind == LMC_SENSOR_PHOTON_RECORD) {  // PhotonRecordSensor::process_hit (detector.cpp:463-485)
+        if constexpr (kBin == 1) continue;
         const unsigned long long slot = atomicAdd(&p.rec_count[0], 1ull);
         if (static_cast<int64_t>(slot) < p.rec_cap) {
           const Quad<R> q10 = get4(f, 10), q11 = get4(f, 11);
@@ -1827,9 +1897,11 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         atomicAdd(&shits[si], 1u);
         continue;
       }
-      if (sd.kind == LMC_SENSOR_STATISTICS) {
-        stats_deposit<R, kAdj>(p, sd, sgrid, events, P.s, arr, w, __int_as_float(iq[I_DEPTH * c]), score);
-        atomicAdd(&shits[si], 1u);
+      if (sd.kind == LMC_SENSOR_STATISTICS) {  // not binned: deposits in the second pass (LDS or atomics)
+        if constexpr (kBin != 1) {
+          stats_deposit<R, kAdj>(p, sd, sgrid, events, P.s, arr, w, __int_as_float(iq[I_DEPTH * c]), score);
+          atomicAdd(&shits[si], 1u);
+        }
         continue;
       }
       int64_t bin;
@@ -1857,17 +1929,17 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
           rn = cmul(rn, pp);
           Erx = cadd(cscale(rm, P.m.x), cscale(rn, P.n.x));
           Ery = cadd(cscale(rm, P.m.y), cscale(rn, P.n.y));
-          atomicAdd(s_cbs, 1u);
+          if constexpr (kBin != 1) atomicAdd(s_cbs, 1u);
         }
-        cbs_deposit<R, kAdj>(p, sd, sgrid, bin, t_idx, Efx, Efy, Erx, Ery, score);
+        cbs_deposit<R, kAdj, kBin>(p, sd, sgrid, bin, t_idx, Efx, Efy, Erx, Ery, score, lbin);
       } else if (sd.kind == LMC_SENSOR_PLANAR_FLUENCE) {
         const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
         const int64_t plane = tplane * sd.n2;
         const cx<R> Ex = cscale(Efx, R(-1));  // lab x negated (detector.cpp:812)
 #pragma unroll 1
         for (int q = 0; q < 4; ++q) {
-          deposit<R, kAdj>(p, sd, sgrid, q * plane + t_idx * tplane + bin, static_cast<double>(stokes(q, Ex, Efy)),
-                           score);
+          deposit<R, kAdj, kBin>(p, sd, sgrid, q * plane + t_idx * tplane + bin,
+                                 static_cast<double>(stokes(q, Ex, Efy)), score, lbin);
         }
       } else if (sd.kind == LMC_SENSOR_PLANAR_FIELD) {
         double sp, cp;
@@ -1878,22 +1950,35 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
         const double eyr = static_cast<double>(Em.r * P.m.y + En.r * P.n.y);
         const double eyi = static_cast<double>(Em.i * P.m.y + En.i * P.n.y);
         const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
-        deposit<R, kAdj>(p, sd, sgrid, 2 * bin, (exr * cp - exi * sp) * wsd, score);
-        deposit<R, kAdj>(p, sd, sgrid, 2 * bin + 1, (exr * sp + exi * cp) * wsd, score);
-        deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin), (eyr * cp - eyi * sp) * wsd, score);
-        deposit<R, kAdj>(p, sd, sgrid, 2 * (plane + bin) + 1, (eyr * sp + eyi * cp) * wsd, score);
+        deposit<R, kAdj, kBin>(p, sd, sgrid, 2 * bin, (exr * cp - exi * sp) * wsd, score, lbin);
+        deposit<R, kAdj, kBin>(p, sd, sgrid, 2 * bin + 1, (exr * sp + exi * cp) * wsd, score, lbin);
+        deposit<R, kAdj, kBin>(p, sd, sgrid, 2 * (plane + bin), (eyr * cp - eyi * sp) * wsd, score, lbin);
+        deposit<R, kAdj, kBin>(p, sd, sgrid, 2 * (plane + bin) + 1, (eyr * sp + eyi * cp) * wsd, score, lbin);
       } else if (sd.kind == LMC_SENSOR_FARFIELD_FLUENCE) {
         const int64_t plane = static_cast<int64_t>(sd.n0) * sd.n1;
 #pragma unroll 1
         for (int q = 0; q < 4; ++q)
-          deposit<R, kAdj>(p, sd, sgrid, q * plane + bin, static_cast<double>(stokes(q, Efx, Efy)), score);
+          deposit<R, kAdj, kBin>(p, sd, sgrid, q * plane + bin, static_cast<double>(stokes(q, Efx, Efy)), score,
+                                 lbin);
       } else {  // LMC_SENSOR_REFLECTANCE
-        deposit<R, kAdj>(p, sd, sgrid, 0, static_cast<double>(w), score);
+        deposit<R, kAdj, kBin>(p, sd, sgrid, 0, static_cast<double>(w), score, lbin);
       }
-      atomicAdd(&shits[si], 1u);
+      if constexpr (kBin != 1) atomicAdd(&shits[si], 1u);
     }
   }
   __syncthreads();
+  if constexpr (kBin == 1) {  // this workgroup's counts, [tile][workgroup]
+    for (int t = tid; t < p.bin_tiles; t += nt) p.bin_pos[static_cast<int64_t>(t) * p.bin_wgs + blockIdx.x] = lbin[t];
+    return;
+  }
+  if constexpr (kBin == 2) {  // each cursor ends where the next workgroup's (or tile's) entries begin
+    for (int t = tid; t < p.bin_tiles; t += nt) {
+      const uint32_t end = blockIdx.x + 1 < static_cast<unsigned>(p.bin_wgs)
+                               ? p.bin_tile[t] + p.bin_pos[static_cast<int64_t>(t) * p.bin_wgs + blockIdx.x + 1]
+                               : p.bin_tile[t + 1];
+      if (lbin[t] != end) atomicOr(p.bin_flag, 1u);
+    }
+  }
   for (int i = tid; i < p.n_sensors; i += nt)
     if (shits[i]) atomicAdd(&p.hits[i], static_cast<unsigned long long>(shits[i]));
   if (tid == 0 && *s_cbs) atomicAdd(&p.stats[4], static_cast<unsigned long long>(*s_cbs));
@@ -1922,6 +2007,115 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
 
 // Plain kernels: defined once, in the host translation unit.
 #ifndef LMC_TRANSPORT_PARTS
+// Binned deposits, step 2: for each tile (one workgroup per tile), the
+// exclusive scan of the per-workgroup counts of pass 1 in place, and the
+// tile's total into tile_tot.
+__global__ void __launch_bounds__(256) k_bin_scan_wgs(uint32_t* pos, uint32_t* tile_tot, int n_wg) {
+  __shared__ uint32_t part[256];
+  uint32_t* row = pos + static_cast<int64_t>(blockIdx.x) * n_wg;
+  const int per = (n_wg + 255) / 256;
+  const int a = threadIdx.x * per, b = a + per < n_wg ? a + per : n_wg;
+  uint32_t sum = 0;
+  for (int i = a; i < b; ++i) sum += row[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < 256; ++k) {
+      const uint32_t v = part[k];
+      part[k] = run;
+      run += v;
+    }
+    tile_tot[blockIdx.x] = run;
+  }
+  __syncthreads();
+  uint32_t run = part[threadIdx.x];
+  for (int i = a; i < b; ++i) {
+    const uint32_t v = row[i];
+    row[i] = run;
+    run += v;
+  }
+}
+
+// Step 3 (one workgroup): tile totals -> first entry of each tile (exclusive
+// scan; tile[n_tiles] = all entries), and the accumulation work items: a tile
+// of c entries becomes ceil(c / chunk) items of at most `chunk` entries (item:
+// tile, first entry, entries, 1 when it is the tile's only item).
+__global__ void __launch_bounds__(1024) k_bin_scan_tiles(uint32_t* tile, uint4* items, uint32_t* n_items, int n_tiles,
+                                                         int chunk, int max_items) {
+  __shared__ uint32_t pe[1024], pi[1024];
+  const int per = (n_tiles + 1023) / 1024;
+  const int a = threadIdx.x * per, b = a + per < n_tiles ? a + per : n_tiles;
+  uint32_t se = 0, si = 0;
+  for (int k = a; k < b; ++k) {
+    se += tile[k];
+    si += (tile[k] + chunk - 1) / chunk;
+  }
+  pe[threadIdx.x] = se;
+  pi[threadIdx.x] = si;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t re = 0, ri = 0;
+    for (int k = 0; k < 1024; ++k) {
+      const uint32_t ve = pe[k], vi = pi[k];
+      pe[k] = re;
+      pi[k] = ri;
+      re += ve;
+      ri += vi;
+    }
+    tile[n_tiles] = re;
+    *n_items = ri < static_cast<uint32_t>(max_items) ? ri : static_cast<uint32_t>(max_items);
+  }
+  __syncthreads();
+  uint32_t re = pe[threadIdx.x], ri = pi[threadIdx.x];
+  for (int k = a; k < b; ++k) {
+    const uint32_t cnt = tile[k];
+    tile[k] = re;
+    const uint32_t nch = (cnt + chunk - 1) / chunk;
+    for (uint32_t j = 0; j < nch; ++j, ++ri) {
+      const uint32_t first = re + j * static_cast<uint32_t>(chunk);
+      const uint32_t len = cnt - j * static_cast<uint32_t>(chunk) < static_cast<uint32_t>(chunk)
+                               ? cnt - j * static_cast<uint32_t>(chunk) : static_cast<uint32_t>(chunk);
+      if (ri < static_cast<uint32_t>(max_items)) items[ri] = make_uint4(static_cast<uint32_t>(k), first, len, nch == 1);
+    }
+    re += cnt;
+  }
+}
+
+// Step 5: sum one work item's entries in LDS (2^shift accumulators of its
+// tile) and add the sums into `out`: a plain read-add-write per cell when the
+// item is its tile's only one (no other workgroup touches the tile), FP64
+// atomics for the cells of a tile split into several items.  Measured on the
+// multilayer run, scattered device FP64 atomics ran at ~1.1e10/s into the
+// 136 MB fluence grid and ~2.4e10/s into an L2-sized grid: the sensor kernel's
+// whole cost.  Entries are read once, coalesced; each tile is written once.
+__global__ void __launch_bounds__(256) k_bin_accumulate(double* out, int64_t n_out, const uint32_t* keys,
+                                                        const double* vals, const uint4* items,
+                                                        const uint32_t* n_items, int shift) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);
+  if (blockIdx.x >= *n_items) return;
+  const uint4 w = items[blockIdx.x];
+  const int T = 1 << shift;
+  const int64_t base = static_cast<int64_t>(w.x) << shift;
+  for (int k = threadIdx.x; k < T; k += blockDim.x) acc[k] = 0.0;
+  __syncthreads();
+  for (uint32_t e = w.y + threadIdx.x; e < w.y + w.z; e += blockDim.x) {
+    const uint32_t k = keys[e] - static_cast<uint32_t>(base);  // < T by construction; checked, not trusted
+    if (k < static_cast<uint32_t>(T))
+      __hip_atomic_fetch_add((lds_double*)(acc + k), vals[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < T && base + k < n_out; k += blockDim.x) {
+    const double v = acc[k];
+    if (v == 0.0) continue;
+    if (w.w)
+      out[base + k] += v;
+    else
+      unsafeAtomicAdd(out + base + k, v);
+  }
+}
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
