@@ -548,12 +548,21 @@ constexpr int kBinShift = 13;
 constexpr int64_t kMaxBinTiles = 16384;
 constexpr int64_t kMaxBinEntries = int64_t(1) << 28;
 constexpr int kBinChunk = 65536;
-// Grids that take binned deposits: the multilayer run's 136 MB planar fluence
-// grid sums 1.9x faster binned (counting, entries, per-tile LDS sums) than with
-// FP64 atomics; for grids up to 32 MB (the polydisperse field and CBS grids,
-// the batch's CBS grids, an L2-sized far-field grid) the atomics were as fast
-// or faster (the batch's many small grids 10x), so they keep them.
+// Which grids take binned deposits.  Measured on the sensor kernel (ms per 5e7
+// photons): the multilayer run, FP64 atomics everywhere 21.3; its 136 MB planar
+// fluence grid binned 15.9; with its 1 MB CBS grid binned too 14.0, with the
+// 518 KB far-field grid as well 14.1.  The polydisperse run, atomics 7.9; its
+// 32 MB field and 4 MB CBS grids binned 9.3, the CBS grid alone 13.6.  The
+// batch's many small CBS grids, 0.96 with atomics and ten times that binned.
+// So a set bins once one of its grids reaches kBinMinBytes (the counting and
+// summing passes then run anyway), and then bins all its grids in HBM.
 constexpr int64_t kBinMinBytes = int64_t(64) << 20;
+
+bool bin_eligible(const SensorDev& o) {
+  return !o.lds && !o.est &&
+         (o.kind == LMC_SENSOR_FARFIELD_CBS || o.kind == LMC_SENSOR_PLANAR_FLUENCE ||
+          o.kind == LMC_SENSOR_FARFIELD_FLUENCE || o.kind == LMC_SENSOR_PLANAR_FIELD);
+}
 
 // Binned HBM deposits one accepted crossing makes into a sensor's grids (only
 // sensors marked for binned deposits have any).
@@ -1248,12 +1257,9 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
         o.lds_off = s->lds_grid;
         s->lds_grid += static_cast<int>(sd.out_len);
       }
-      // grids far beyond the caches take binned deposits (forward runs)
-      o.bin = (!o.lds && !o.est && sd.out_len * 8 >= kBinMinBytes &&
-               (sd.kind == LMC_SENSOR_FARFIELD_CBS || sd.kind == LMC_SENSOR_PLANAR_FLUENCE ||
-                sd.kind == LMC_SENSOR_FARFIELD_FLUENCE || sd.kind == LMC_SENSOR_PLANAR_FIELD))
-                  ? 1
-                  : 0;
+      // binned deposits (forward runs): a grid far beyond the caches; then
+      // the set's other HBM grids too (finish_set)
+      o.bin = (bin_eligible(o) && sd.out_len * 8 >= kBinMinBytes) ? 1 : 0;
       s->sensors_host.push_back(o);
       s->sensor_scene.push_back(scene_idx);
       P.count++;
@@ -1283,6 +1289,10 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
 }
 
 int finish_set(lmc_scene* s) {
+  bool any_bin = false;
+  for (const SensorDev& o : s->sensors_host) any_bin = any_bin || o.bin;
+  if (any_bin)
+    for (SensorDev& o : s->sensors_host) o.bin = bin_eligible(o) ? 1 : 0;
   int rc = s->sensors_dev.upload(s->sensors_host.data(), s->sensors_host.size());
   if (!rc) rc = s->planes_dev.upload(s->planes_host.data(), s->planes_host.size());
   if (!rc) rc = s->out.alloc(std::max<int64_t>(1, s->n_out));
