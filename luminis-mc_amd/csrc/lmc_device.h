@@ -310,14 +310,13 @@ template <class R> struct LayerDev {
   int32_t sp_first, sp_count, sel_off, nbr;
   int32_t pl_first, pl_count;
   R z_top;
-  R n_lo;  // n - (R)n: with n, the refractive index to ~48 bits for the FP64 optical path
+  R pad_;
+  double n64;  // the refractive index in FP64, for the FP64 optical path
 };
-// Refractive index of a layer in FP64 (float-float for FP32 scenes): the
-// optical path, whose k0 multiple is the planar field's phase, accumulates in
-// FP64 without the ~3e-8 relative bias of an FP32 index.
-template <class R> __device__ __forceinline__ double layer_n64(const LayerDev<R>& L) {
-  return static_cast<double>(L.n) + static_cast<double>(L.n_lo);
-}
+// Refractive index of a layer in FP64 (stored so in both builds): the optical
+// path, whose k0 multiple is the planar field's phase, accumulates in FP64
+// without the ~3e-8 relative bias of an FP32 index.
+template <class R> __device__ __forceinline__ double layer_n64(const LayerDev<R>& L) { return L.n64; }
 
 struct SensorDev {
   int32_t kind, absorb, plane, id;

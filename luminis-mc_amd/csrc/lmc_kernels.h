@@ -1255,7 +1255,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       on_iface = sizeof(R) == 4;
       ph.layer += up ? 1 : -1;
       const R mt = v.L[ph.layer].mu_t;
-      step = mt > R(0) ? rem / mt : rem;
+      step = mt > R(0) ? qdiv(rem, mt) : rem;
     }
     // Sample::is_inside (sample.cpp:268-288)
     if (!(ph.pos.z >= R(0)) || ph.pos.z >= v.L[ph.layer].z_top || !isfinite(ph.pos.x) ||

@@ -326,7 +326,8 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.mu_t = static_cast<R>(d.mu_t);
     o.albedo = static_cast<R>(d.mu_t > 0 ? d.mu_s / d.mu_t : 1.0);
     o.n = static_cast<R>(d.n);
-    o.n_lo = static_cast<R>(d.n - static_cast<double>(o.n));
+    o.pad_ = R(0);
+    o.n64 = d.n;
     o.mu_s = static_cast<R>(d.mu_s);
     o.mu_a = static_cast<R>(d.mu_a);
     o.sp_first = d.species_first;
