@@ -316,7 +316,7 @@ int lmc_scene_zero(lmc_scene* s);
 /* PhotonRecordSensor records gathered since creation or the last zero, in
  * crossing-queue order (not photon order): copies min(max_records, count)
  * records of LMC_RECORD_FIELDS doubles into `out` (may be NULL to count
- * only) and sets *n_records to the count.  Single-scene forward runs only.
+ * only) and sets *n_records to the count.  Forward runs only.
  * Replaces: PhotonRecordSensor::recorded_photons (detector.hpp:280-298) and
  * its merge_from (detector.cpp:455-461). */
 int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n_records);
@@ -328,19 +328,25 @@ int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n
  * (production or xoshiro256++ check mode) as photon j of lmc_scene_run on
  * that scene alone with the same seed,
  * so a batched scene reproduces its single-scene run (up to the summation
- * order of the FP64 accumulators). */
+ * order of the FP64 accumulators).  Absorption grids and PhotonRecordSensor
+ * records are kept per scene; the CBS estimator mode runs single scenes
+ * only (LMC_ERR_UNSUPPORTED). */
 typedef struct lmc_batch lmc_batch;
 int lmc_batch_create(int32_t n_scenes, const lmc_scene_desc* scenes, lmc_batch** out);
 void lmc_batch_destroy(lmc_batch* b);
 /* Run budgets[i] photons of scene i (photon indices photon_offset + [0,
  * budgets[i]), below 2^32; run->n_photons is ignored) and add into the
  * per-scene device accumulators.  use_graph != 0: the launch sequence is
- * captured once into a hipGraph and replayed while budgets and run repeat. */
+ * captured once into a hipGraph and replayed while budgets and run repeat
+ * (a batch with PhotonRecordSensor records runs eagerly: its record list is
+ * sized and gathered on the host around each sensor launch). */
 int lmc_batch_run(lmc_batch* b, const uint64_t* budgets, const lmc_run_desc* run,
                   int32_t use_graph, lmc_stats* stats);
 /* Add scene `scene`'s accumulators / hit counters into host buffers, as
  * lmc_scene_read. */
 int lmc_batch_read(lmc_batch* b, int32_t scene, double* out, int64_t n, uint64_t* hits);
+/* Scene `scene`'s PhotonRecordSensor records, as lmc_scene_records. */
+int lmc_batch_records(lmc_batch* b, int32_t scene, double* out, int64_t max_records, int64_t* n_records);
 int lmc_batch_zero(lmc_batch* b);
 
 /* ---- adjoint (path-length / event-count score) ------------------------ */

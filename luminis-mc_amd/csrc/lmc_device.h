@@ -310,8 +310,18 @@ template <class R> struct LayerDev {
   int32_t sp_first, sp_count, sel_off, nbr;
   int32_t pl_first, pl_count;
   R z_top;
-  R pad_;
+  int32_t scene;  // index of the layer's scene in its set (a batch's per-scene Absorption grid)
+  int32_t pad_;
   double n64;  // the refractive index in FP64, for the FP64 optical path
+};
+
+// One scene's Absorption grid (Absorption::record_absorption,
+// absortion.cpp:47-77) for batched runs: r-z(-t) cells from `off` in the
+// accumulators; on == 0 when the scene records none.
+template <class R> struct AbsDev {
+  R radius, depth, dr, dz, dt;
+  int32_t nr, nz, nt, on;
+  int64_t off;
 };
 // Refractive index of a layer in FP64 (stored so in both builds): the optical
 // path, whose k0 multiple is the planar field's phase, accumulates in FP64

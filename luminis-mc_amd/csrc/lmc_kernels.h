@@ -66,6 +66,7 @@ template <class R> struct KParams {
   const LaunchConst<R>* lcs;     // one per scene
   const uint64_t* prefix;        // batched runs: first global photon index of each scene, n_scenes + 1
   const R* aux;                  // estimator bin geometry and statistics histogram parameters
+  const AbsDev<R>* abs_scenes;   // batched runs: each scene's Absorption grid
   R abs_radius, abs_depth, abs_dr, abs_dz, abs_dt;  // Absorption grid of a single-scene run
   int32_t abs_on, abs_nr, abs_nz, abs_nt;
   int64_t abs_off;
@@ -1455,18 +1456,25 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   const R w_in = ph.w;
   if (Ld.mu_t > R(0)) ph.w *= Ld.albedo;
   if constexpr (kAux == 2) {  // Absorption::record_absorption (absortion.cpp:47-77) of w * mu_a / mu_t
+    // a single scene's grid from the kernel arguments; a batch's per scene
+    AbsDev<R> A;
+    if constexpr (kBatch) {
+      A = p.abs_scenes[Ld.scene];
+    } else {
+      A = {p.abs_radius, p.abs_depth, p.abs_dr, p.abs_dz, p.abs_dt, p.abs_nr, p.abs_nz, p.abs_nt, 1, p.abs_off};
+    }
     const R d_w = Ld.mu_t > R(0) ? w_in * (Ld.mu_a / Ld.mu_t) : R(1);
     const R r = M<R>::sqrt(ph.pos.x * ph.pos.x + ph.pos.y * ph.pos.y), z = ph.pos.z;
-    if (!(r > p.abs_radius) && z >= R(0) && !(z > p.abs_depth)) {
-      const int ir = static_cast<int>(r / p.abs_dr), iz = static_cast<int>(z / p.abs_dz);
-      if (ir < p.abs_nr && iz < p.abs_nz) {
-        double* g = p.out + p.abs_off;
-        const int64_t cell = static_cast<int64_t>(ir) * p.abs_nz + iz;
+    if (A.on && !(r > A.radius) && z >= R(0) && !(z > A.depth)) {
+      const int ir = static_cast<int>(r / A.dr), iz = static_cast<int>(z / A.dz);
+      if (ir < A.nr && iz < A.nz) {
+        double* g = p.out + A.off;
+        const int64_t cell = static_cast<int64_t>(ir) * A.nz + iz;
         unsafeAtomicAdd(g + cell, static_cast<double>(d_w));
-        if (p.abs_dt > R(0)) {
-          const int k = static_cast<int>(M<R>::floor((ph.t0 + ph.gpl) / p.abs_dt)) + 1;
-          if (k >= 1 && k < p.abs_nt)
-            unsafeAtomicAdd(g + static_cast<int64_t>(k) * p.abs_nr * p.abs_nz + cell, static_cast<double>(d_w));
+        if (A.dt > R(0)) {
+          const int k = static_cast<int>(M<R>::floor((ph.t0 + ph.gpl) / A.dt)) + 1;
+          if (k >= 1 && k < A.nt)
+            unsafeAtomicAdd(g + static_cast<int64_t>(k) * A.nr * A.nz + cell, static_cast<double>(d_w));
         }
       }
     }
@@ -1541,7 +1549,7 @@ template <class R> struct LdsHdr {
 // LDS for tables that two workgroups could not hold).
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false, int kAux = 1>
 __device__ __forceinline__ void transport_body(const KParams<R>& p) {
-  static_assert(kAux != 2 || (!kBatch && !kAdj), "the Absorption grid is recorded by single-scene forward runs");
+  static_assert(kAux != 2 || !kAdj, "the Absorption grid is recorded by forward runs");
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1878,7 +1886,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
           put(4, static_cast<double>(arr));
           put(5, static_cast<double>(q11.d));
           put(6, static_cast<double>(w));
-          put(7, 0.0);  // k: filled by the host
+          put(7, static_cast<double>(si));  // the host replaces it with the scene's k
           put3(8, q10.a, q10.b, q10.c);
           put3(11, q11.a, q11.b, q11.c);
           put(14, static_cast<double>(hx));

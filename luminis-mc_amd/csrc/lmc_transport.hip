@@ -109,6 +109,7 @@ template <class R> struct DevTables {
   DBuf<unsigned char> tab;  // CDFs, guide tables, S1/S2 nodes, selection CDFs (KParams::tab)
   DBuf<LaunchConst<R>> lcs;
   DBuf<R> aux;
+  DBuf<AbsDev<R>> abs;  // per-scene Absorption grids (batched runs)
   size_t tab_bytes = 0;
 };
 
@@ -122,6 +123,7 @@ struct SceneInfo {
   int n_sensors = 0;
   int n_single_planes = 0;     // planes a photon crosses at most once (records <= 1)
   int n_multi_planes = 0;      // interior non-absorbing planes (records <= max_events + 2)
+  double k_wave = 0.0;         // 2 pi / wavelength of the scene's laser (records)
   lmc_laser_desc laser{};
   lmc_absorption_desc absorption{};  // out_offset relative to out_base
 };
@@ -157,12 +159,12 @@ struct lmc_scene {
   // some scene's stack, or through its launch point
   bool flight_planes = false;
   // PhotonRecordSensor records: a device list per sensor-kernel launch,
-  // gathered on the host; rec_per_crossing = most record sensors on one plane
+  // gathered on the host into one list per scene; rec_per_crossing = most
+  // record sensors on one plane
   int rec_per_crossing = 0;
-  double rec_k = 0.0;  // 2 pi / wavelength of the scene's laser
   DBuf<double> rec;
   DBuf<unsigned long long> rec_count;  // [0] written, [1] no room
-  std::vector<double> rec_host;
+  std::vector<std::vector<double>> rec_host;
   DBuf<double> out;
   DBuf<unsigned long long> hits;
   DBuf<unsigned long long> counter;  // [0] work counter, [1..5] stats
@@ -326,7 +328,7 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.mu_t = static_cast<R>(d.mu_t);
     o.albedo = static_cast<R>(d.mu_t > 0 ? d.mu_s / d.mu_t : 1.0);
     o.n = static_cast<R>(d.n);
-    o.pad_ = R(0);
+    o.scene = s->layer_scene[i];
     o.n64 = d.n;
     o.mu_s = static_cast<R>(d.mu_s);
     o.mu_a = static_cast<R>(d.mu_a);
@@ -364,7 +366,25 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     c.t_off = Ls.time_offset;
     c.launch_layer = s->scenes[k].launch_layer;
   }
+  std::vector<AbsDev<R>> ab(s->scenes.size());
+  for (size_t k = 0; k < s->scenes.size(); ++k) {
+    const lmc_absorption_desc& A = s->scenes[k].absorption;
+    AbsDev<R>& a = ab[k];
+    std::memset(&a, 0, sizeof(a));
+    if (!A.enabled) continue;
+    a.on = 1;
+    a.nr = A.n_r;
+    a.nz = A.n_z;
+    a.nt = A.n_t;
+    a.radius = static_cast<R>(A.radius);
+    a.depth = static_cast<R>(A.depth);
+    a.dr = static_cast<R>(A.d_r);
+    a.dz = static_cast<R>(A.d_z);
+    a.dt = static_cast<R>(A.d_t);
+    a.off = s->scenes[k].out_base + A.out_offset;
+  }
   int rc;
+  if ((rc = T.abs.upload(ab.data(), ab.size()))) return rc;
   if ((rc = T.lcs.upload(lc.data(), lc.size()))) return rc;
   {
     std::vector<R> eg(s->aux_host.begin(), s->aux_host.end());
@@ -600,8 +620,10 @@ int64_t records_per_photon(const SceneInfo& sc, int max_events) {
 // PhotonRecordSensor records of one launch: once the transport kernel has
 // filled the crossing queue, size the device list for every queued crossing
 // times the record sensors of one plane (so no record can miss a slot), and
-// clear its counters; after the sensor kernel, append the records to the
-// scene's host list with the wave number filled in.
+// clear its counters; after the sensor kernel, append each record to its
+// scene's host list.  The device writes the record's absolute sensor index
+// into field 7; the host maps it to the scene and puts the scene's wave
+// number there.
 template <class R> int prepare_records(lmc_scene* s, KParams<R>& p) {
   LMC_HIP(hipEventSynchronize(s->ev1));
   unsigned long long nq = 0;
@@ -623,12 +645,20 @@ int gather_records(lmc_scene* s) {
   unsigned long long cnt[2];
   LMC_HIP(hipMemcpy(cnt, s->rec_count.p, sizeof(cnt), hipMemcpyDeviceToHost));
   if (cnt[1] != 0) return fail(LMC_ERR_DEVICE, "PhotonRecordSensor list overflowed");
-  const size_t n = static_cast<size_t>(cnt[0]), base = s->rec_host.size();
+  const size_t n = static_cast<size_t>(cnt[0]);
   if (n == 0) return LMC_OK;
-  s->rec_host.resize(base + n * LMC_RECORD_FIELDS);
-  LMC_HIP(hipMemcpy(s->rec_host.data() + base, s->rec.p, sizeof(double) * n * LMC_RECORD_FIELDS,
-                    hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < n; ++i) s->rec_host[base + i * LMC_RECORD_FIELDS + 7] = s->rec_k;
+  std::vector<double> tmp(n * LMC_RECORD_FIELDS);
+  LMC_HIP(hipMemcpy(tmp.data(), s->rec.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  s->rec_host.resize(s->scenes.size());
+  for (size_t i = 0; i < n; ++i) {
+    double* rec = tmp.data() + i * LMC_RECORD_FIELDS;
+    const double sid = rec[7];
+    if (!(sid >= 0.0 && sid < static_cast<double>(s->sensor_scene.size())))
+      return fail(LMC_ERR_DEVICE, "PhotonRecordSensor record with a bad sensor index");
+    const int k = s->sensor_scene[static_cast<size_t>(sid)];
+    rec[7] = s->scenes[k].k_wave;
+    s->rec_host[k].insert(s->rec_host[k].end(), rec, rec + LMC_RECORD_FIELDS);
+  }
   return LMC_OK;
 }
 
@@ -651,7 +681,10 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   p.planes = s->planes_dev.p;
   p.lcs = T.lcs.p;
   p.aux = T.aux.p;  // estimator bin geometry, statistics histogram parameters
-  if (!s->scenes.empty() && s->scenes[0].absorption.enabled && s->scenes.size() == 1) {
+  p.abs_scenes = T.abs.p;
+  if (batch) {
+    for (const SceneInfo& sc : s->scenes) p.abs_on |= sc.absorption.enabled ? 1 : 0;
+  } else if (!s->scenes.empty() && s->scenes[0].absorption.enabled) {
     const lmc_absorption_desc& A = s->scenes[0].absorption;
     p.abs_on = 1;
     p.abs_nr = A.n_r;
@@ -744,6 +777,9 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   if (rec_run && scratch)
     return fail(LMC_ERR_UNSUPPORTED, "no-alloc runs keep no PhotonRecordSensor list (it is sized per launch): "
                                      "use lmc_scene_run");
+  // the record list is sized and gathered on the host around every sensor
+  // launch, so a set with record sensors runs eagerly
+  if (rec_run) use_graph = false;
   const bool track = r->track_reverse_paths != 0;
   bool has_stats = false;
   for (const SensorDev& sd : s->sensors_host) has_stats = has_stats || sd.kind == LMC_SENSOR_STATISTICS;
@@ -1133,12 +1169,10 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
   sc.launch_layer = sc.layer_base + ll;
   sc.out_base = s->n_out;
   sc.n_out = d->n_out;
-  if (d->absorption.enabled && !privatise)
-    return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by single-scene runs only");
   sc.absorption = d->absorption;
+  sc.k_wave = 2.0 * M_PI / d->laser.wavelength;
   sc.n_sensors = d->n_sensors;
   sc.laser = d->laser;
-  if (s->scenes.empty()) s->rec_k = 2.0 * M_PI / d->laser.wavelength;
   const int64_t tbase = static_cast<int64_t>(s->tables.size());
   const int scene_idx = static_cast<int>(s->scenes.size());
   if (d->n_tables) s->tables.insert(s->tables.end(), d->tables, d->tables + d->n_tables);
@@ -1266,7 +1300,6 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
       P.count++;
       if (sd.kind == LMC_SENSOR_FARFIELD_CBS && !o.est) P.flags |= kPlaneCbs;
       if (sd.kind == LMC_SENSOR_PHOTON_RECORD) {
-        if (!privatise) return fail(LMC_ERR_UNSUPPORTED, "PhotonRecordSensor records are kept by single-scene runs only");
         P.flags |= kPlaneRec;
         ++n_rec_plane;
       }
@@ -1350,6 +1383,17 @@ void fold_time_bins(const lmc_scene* s, int k, double* buf) {
         for (int64_t i = 0; i < tplane; ++i) g[i] += g[t * tplane + i];
     }
   }
+}
+
+// Scene k's PhotonRecordSensor records: min(max_records, count) of them into
+// `out` (may be null), the count into *n_records.
+int copy_records(const lmc_scene* s, int k, double* out, int64_t max_records, int64_t* n_records) {
+  const std::vector<double>* v = static_cast<size_t>(k) < s->rec_host.size() ? &s->rec_host[k] : nullptr;
+  const int64_t n = v ? static_cast<int64_t>(v->size() / LMC_RECORD_FIELDS) : 0;
+  if (n_records) *n_records = n;
+  if (v && out && max_records > 0)
+    std::memcpy(out, v->data(), sizeof(double) * LMC_RECORD_FIELDS * std::min(n, max_records));
+  return LMC_OK;
 }
 
 // Add scene k's accumulators and hit counters into caller buffers.
@@ -1471,11 +1515,7 @@ int lmc_scene_read(lmc_scene* s, double* out, int64_t n, uint64_t* hits) {
 int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n_records) {
   if (!s) return fail(LMC_ERR_INVALID, "null scene");
   if (max_records < 0) return fail(LMC_ERR_INVALID, "max_records must be >= 0");
-  const int64_t n = static_cast<int64_t>(s->rec_host.size() / LMC_RECORD_FIELDS);
-  if (n_records) *n_records = n;
-  if (out && max_records > 0)
-    std::memcpy(out, s->rec_host.data(), sizeof(double) * LMC_RECORD_FIELDS * std::min(n, max_records));
-  return LMC_OK;
+  return copy_records(s, 0, out, max_records, n_records);
 }
 
 int lmc_scene_run_adjoint(lmc_scene* s, const lmc_run_desc* r, double* grad, int64_t n_grad, lmc_stats* st) {
@@ -1543,6 +1583,14 @@ int lmc_batch_read(lmc_batch* b, int32_t scene, double* out, int64_t n, uint64_t
   if (scene < 0 || scene >= static_cast<int32_t>(b->set.scenes.size()))
     return fail(LMC_ERR_INVALID, "scene index out of range");
   return read_scene(&b->set, scene, out, n, hits);
+}
+
+int lmc_batch_records(lmc_batch* b, int32_t scene, double* out, int64_t max_records, int64_t* n_records) {
+  if (!b) return fail(LMC_ERR_INVALID, "null batch");
+  if (scene < 0 || scene >= static_cast<int32_t>(b->set.scenes.size()))
+    return fail(LMC_ERR_INVALID, "scene index out of range");
+  if (max_records < 0) return fail(LMC_ERR_INVALID, "max_records must be >= 0");
+  return copy_records(&b->set, scene, out, max_records, n_records);
 }
 
 int lmc_batch_zero(lmc_batch* b) {

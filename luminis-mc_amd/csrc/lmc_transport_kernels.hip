@@ -17,8 +17,8 @@ namespace {
 // The transport kernel of a launch: the estimator, 1024-thread or
 // 512/256-thread body; a batch or a single scene; with or without reverse-path
 // tracking; tables in LDS or read through the caches.  kAux as in the
-// kernels: 2 (Absorption grid) for single-scene forward runs only, 0 for FP32
-// Philox forward runs only; nullptr where a combination does not exist.
+// kernels: 2 (Absorption grid) for forward runs only, 0 for FP32 Philox
+// forward runs only; nullptr where a combination does not exist.
 template <class R, bool kXo, bool kAdj, int kAux>
 TransportFn<R> pick_transport(bool est, bool wide, bool lds_tables, bool batch, bool track) {
   if constexpr ((kAux == 2 && kAdj) || (kAux == 0 && (sizeof(R) != 4 || kXo || kAdj))) {
@@ -29,21 +29,17 @@ TransportFn<R> pick_transport(bool est, bool wide, bool lds_tables, bool batch, 
       else return lds_tables ? k_transport_est<R, kXo, true, kAux> : k_transport_est<R, kXo, false, kAux>;
     }
     if (batch) {
-      if constexpr (kAux == 2) {
-        return nullptr;
-      } else {
-        if (wide) {
-          if constexpr (sizeof(R) == 4 && !kAdj)
-            return track ? k_transport_wide<R, kXo, true, kAdj, true, kAux>
-                         : k_transport_wide<R, kXo, false, kAdj, true, kAux>;
-          else return nullptr;
-        }
-        if (track)
-          return lds_tables ? k_transport<R, kXo, true, true, kAdj, true, kAux>
-                            : k_transport<R, kXo, true, false, kAdj, true, kAux>;
-        return lds_tables ? k_transport<R, kXo, false, true, kAdj, true, kAux>
-                          : k_transport<R, kXo, false, false, kAdj, true, kAux>;
+      if (wide) {
+        if constexpr (sizeof(R) == 4 && !kAdj)
+          return track ? k_transport_wide<R, kXo, true, kAdj, true, kAux>
+                       : k_transport_wide<R, kXo, false, kAdj, true, kAux>;
+        else return nullptr;
       }
+      if (track)
+        return lds_tables ? k_transport<R, kXo, true, true, kAdj, true, kAux>
+                          : k_transport<R, kXo, true, false, kAdj, true, kAux>;
+      return lds_tables ? k_transport<R, kXo, false, true, kAdj, true, kAux>
+                        : k_transport<R, kXo, false, false, kAdj, true, kAux>;
     }
     if (wide) {
       if constexpr (sizeof(R) == 4 && !kAdj)

@@ -198,6 +198,7 @@ def lib():
     L.lmc_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(c_uint64), ctypes.POINTER(RunDesc), c_int32,
                                 ctypes.POINTER(Stats)]
     L.lmc_batch_read.argtypes = [ctypes.c_void_p, c_int32, c_dp, c_int64, ctypes.POINTER(c_uint64)]
+    L.lmc_batch_records.argtypes = [ctypes.c_void_p, c_int32, c_dp, c_int64, ctypes.POINTER(c_int64)]
     L.lmc_batch_zero.argtypes = [ctypes.c_void_p]
     L.lmc_mie_amplitudes.argtypes = [c_double, c_double, c_double, c_int32, c_dp, c_dp, c_dp]
     L.lmc_mie_info.argtypes = [c_double, c_double, c_double, c_dp, c_dp, c_dp]

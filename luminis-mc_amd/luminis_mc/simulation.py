@@ -309,15 +309,26 @@ class GpuBatch:
     persistent launch (lmc_batch_*).  Scene i's photon j follows the same
     random stream as photon j of a GpuScene run of that scene alone."""
 
-    def __init__(self, scenes):
-        """scenes: iterable of (sample, laser, sensors) triples."""
-        self.builds = [SceneBuild(sa, la, list(se)) for sa, la, se in scenes]
+    def __init__(self, scenes, builds=None):
+        """scenes: iterable of (sample, laser, sensors) or (sample, laser,
+        sensors, absorption) tuples; absorption is an Absorption grid or None."""
+        self.builds = list(builds) if builds is not None else []
+        for t in (scenes if builds is None else ()):
+            if len(t) not in (3, 4):
+                raise ValueError("a batch scene is (sample, laser, sensors[, absorption])")
+            sa, la, se = t[:3]
+            self.builds.append(SceneBuild(sa, la, list(se), t[3] if len(t) == 4 else None))
         if not self.builds:
             raise ValueError("a batch needs at least one scene")
         arr = (_lib.SceneDesc * len(self.builds))(*[b.desc for b in self.builds])
         self._descs = arr
         self._h = ctypes.c_void_p()
         _lib.check(_lib.lib().lmc_batch_create(len(self.builds), arr, ctypes.byref(self._h)), "lmc_batch_create")
+
+    @classmethod
+    def from_builds(cls, builds) -> "GpuBatch":
+        """A device batch from already flattened descriptors (SceneBuild or alike)."""
+        return cls(None, builds=builds)
 
     def __len__(self):
         return len(self.builds)
@@ -341,13 +352,26 @@ class GpuBatch:
                    "lmc_batch_read")
         return out, hits
 
+    def records(self, k: int) -> np.ndarray:
+        """Scene k's PhotonRecordSensor records: an (n, RECORD_FIELDS) array,
+        column 0 the sensor's index in scene k."""
+        n = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.lmc_batch_records(self._h, int(k), None, 0, ctypes.byref(n)), "lmc_batch_records")
+        rows = np.zeros((n.value, _lib.RECORD_FIELDS))
+        if n.value:
+            _lib.check(L.lmc_batch_records(self._h, int(k), _lib.dptr(rows), n.value, ctypes.byref(n)),
+                       "lmc_batch_records")
+        return rows
+
     def zero(self) -> None:
         _lib.check(_lib.lib().lmc_batch_zero(self._h), "lmc_batch_zero")
 
     def merge_into_sensors(self) -> None:
         for k, bld in enumerate(self.builds):
             out, hits = self.read(k)
-            bld.scatter_results(out, hits)
+            has_rec = any(s.kind == _lib.SENSOR_PHOTON_RECORD for s in bld.sensors)
+            bld.scatter_results(out, hits, self.records(k) if has_rec else None)
 
     def close(self) -> None:
         if self._h:
@@ -371,8 +395,6 @@ def run_simulation_batched(configs, use_graph: bool = True) -> dict:
         raise ValueError("no configs")
     for c in configs:
         validate_sim_config(c)
-        if c.absorption is not None:
-            raise NotImplementedError("the Absorption grid is recorded by single-scene runs only")
     keys = ("seed", "MAX_EVENTS", "track_reverse_paths", "precision", "rng", "photon_offset",
             "block_size", "blocks_per_cu", "claim")
     c0 = configs[0]
@@ -380,7 +402,7 @@ def run_simulation_batched(configs, use_graph: bool = True) -> dict:
         for k in keys:
             if getattr(c, k) != getattr(c0, k):
                 raise ValueError(f"batched configs must share {k}")
-    batch = GpuBatch([(c.sample, c.laser, c.detector.detectors) for c in configs])
+    batch = GpuBatch([(c.sample, c.laser, c.detector.detectors, c.absorption) for c in configs])
     try:
         st = batch.run([int(c.n_photons) for c in configs], run_desc_from(c0, n_photons=0), use_graph)
         batch.merge_into_sensors()

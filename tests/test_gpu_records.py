@@ -7,8 +7,8 @@
 * FP32 production path: counts, weight sums against a reflectance sensor,
   frames, the rebuilt CBS grid, and a run long enough to span several
   transport launches (record lists gathered per launch).
-* run_simulation_parallel fills recorded_photons; batched and gradient runs
-  refuse record sensors.
+* run_simulation_parallel fills recorded_photons; gradient runs refuse record
+  sensors (batched runs keep them per scene: test_gpu_batch_extras.py).
 """
 import math
 
@@ -19,7 +19,7 @@ import luminis_mc as lmc
 import oracle
 import records_check as rc
 import scenes
-from luminis_mc.simulation import GpuBatch, GpuScene, SceneBuild, SimConfig, run_desc_from
+from luminis_mc.simulation import GpuScene, SceneBuild, SimConfig, run_desc_from
 from test_oracle_cpu import _record_scene
 
 pytestmark = pytest.mark.gpu
@@ -145,9 +145,3 @@ def test_record_sensor_refusals(require_gpu):
     with pytest.raises(NotImplementedError):
         sc.run_adjoint(_rd("fp32", "philox", 1000, 5))
     sc.close()
-    m, ph = scenes.mie_medium(0.5e-3, 1.59, 1.33, 10.0, 0.1)
-    sample = lmc.Sample(1.33)
-    sample.add_layer(m, 0.0, math.inf)
-    laser = lmc.Laser(1.0, 0.0, scenes.WAVELENGTH, 0.0, lmc.LaserSource.Point)
-    with pytest.raises(NotImplementedError):
-        GpuBatch([(sample, laser, [lmc.PhotonRecordSensor(0.0)])])
