@@ -280,7 +280,9 @@ template <class R> struct SpeciesDev {
   R cdf_min, cdf_step, hg_g, inv_ds;
   R pi_inorm;  // pi * I_norm, the estimator's angular normalisation (detector.cpp:105-118, 1436)
   R alb;       // mu_s / (mu_a + mu_s) of the species (estimator weight, detector.cpp:1466)
+  R pad_[2];   // 64 bytes in FP32: the event loop's species address is a shift
 };
+static_assert(sizeof(SpeciesDev<float>) == 64, "FP32 species record: 64 bytes");
 
 // S1/S2 at one node: (s1.re, s1.im, s2.re, s2.im).
 template <class R> struct STab;
@@ -303,17 +305,18 @@ __device__ __forceinline__ void stab_get(const STab<double>& t, cx<double>& s1, 
 // several scenes can share one array: neighbour flags (bit 0: a layer above in
 // +z, bit 1: a layer below), the scene's top face (+inf when semi-infinite)
 // and the scene's sensor planes [pl_first, pl_first + pl_count).
+// 64 bytes in FP32 (a power of two: the event loop's layer addresses are a
+// shift, not a multiply).  The refractive index is kept in FP64 only.
 template <class R> struct LayerDev {
   R z_min, z_max;  // z_max = +inf for a semi-infinite last layer
-  R mfp, mu_t, albedo, n;
+  R mfp, mu_t, albedo, z_top;
   R mu_s, mu_a;
-  int32_t sp_first, sp_count, sel_off, nbr;
-  int32_t pl_first, pl_count;
-  R z_top;
+  int32_t sp_first, sp_count, sel_off, pl_first;
+  int16_t nbr, pl_count;
   int32_t scene;  // index of the layer's scene in its set (a batch's per-scene Absorption grid)
-  int32_t pad_;
   double n64;  // the refractive index in FP64, for the FP64 optical path
 };
+static_assert(sizeof(LayerDev<float>) == 64, "FP32 layer record: 64 bytes");
 
 // One scene's Absorption grid (Absorption::record_absorption,
 // absortion.cpp:47-77) for batched runs: r-z(-t) cells from `off` in the
@@ -337,7 +340,10 @@ struct SensorDev {
   double g[8];
   int64_t out_off;
   int32_t lds_off;
-  int32_t bin;  // forward runs: deposits binned by tile and summed in LDS (grids far beyond the caches)
+  // forward runs: deposits binned by tile and summed in LDS (grids far beyond
+  // the caches); bin_d the same plan for runs whose transport kernel deposits
+  // the direct sensors itself (those sensors then take no queued deposits)
+  int32_t bin, bin_d;
   // estimator mode: est != 0 takes no crossings; a FARFIELD_CBS sensor then
   // deposits every vertex's forced scatter into its first est_imax theta rows
   // (bin geometry at aux in the auxiliary pool), in the transport kernel's
@@ -347,7 +353,11 @@ struct SensorDev {
   // conservative grid bounds for the transport kernel's queueing test: exit
   // cosine -s_z >= q_cmin, hit point in [q_x0, q_x1] x [q_y0, q_y1], arrival
   // in [q_t0, q_t1] (slightly wider than the grid; the sensor kernel bins exactly)
-  float q_cmin, q_x0, q_x1, q_y0, q_y1, q_t0, q_t1, pad3;
+  float q_cmin, q_x0, q_x1, q_y0, q_y1, q_t0, q_t1;
+  // a light sensor (planar fluence, far-field fluence, reflectance): the
+  // transport kernels of forward runs that deposit_direct() names deposit
+  // its crossings themselves, which are then never queued
+  int32_t direct, pad4;
 };
 
 // flags: kPlaneCbs a FarFieldCBSSensor (analog) reads the reverse-path

@@ -70,7 +70,8 @@ template <class R> struct KParams {
   R abs_radius, abs_depth, abs_dr, abs_dz, abs_dt;  // Absorption grid of a single-scene run
   int32_t abs_on, abs_nr, abs_nz, abs_nt;
   int64_t abs_off;
-  int32_t est_lds_n, est_pad;    // doubles of the estimator grids staged in LDS
+  int32_t est_lds_n;             // doubles of the estimator grids staged in LDS
+  int32_t direct;                // 1: the transport kernel deposits the direct sensors (deposits_direct)
   R pz_lo, pz_hi;                // lowest and highest sensor plane (as R) of all scenes of the launch
   // PhotonRecordSensor records of one sensor-kernel launch (LMC_RECORD_FIELDS
   // doubles each), [0] records written, [1] records that found no room
@@ -146,6 +147,7 @@ template <class R> struct SceneView {
   const PlaneDev* planes;
   double* lgrid;     // LDS accumulators
   unsigned* lhits;   // LDS hit counters [kMaxSensors], then crossings, cbs
+  double* lrefl;     // single scenes: LDS reflectance sums of the direct sensors [kMaxSensors]
   unsigned long long* wq_base;  // per wave: first slot of its current queue chunk
   unsigned* wq_used;            // per wave: slots used in that chunk
 };
@@ -490,7 +492,7 @@ __device__ __forceinline__ void deposit(const KParams<R>& p, const SensorDev& sd
                                         double val, const R* score, unsigned* lbin = nullptr) {
   if constexpr (kBin != 0) {
     static_assert(!kAdj, "binned deposits serve forward runs");
-    if (!sd.bin) {  // LDS copy or FP64 atomic, in the writing pass only
+    if (!(p.direct ? sd.bin_d : sd.bin)) {  // LDS copy or FP64 atomic, in the writing pass only
       if constexpr (kBin == 2) acc_add(lds_grid + sd.lds_off, p.out + sd.out_off, sd.lds != 0, off, val);
       return;
     }
@@ -683,12 +685,73 @@ __device__ __forceinline__ bool sensor_bin(const SensorDev& sd, R hx, R hy, R ar
   return kind == LMC_SENSOR_REFLECTANCE;
 }
 
+// Transport kernels that deposit the light sensors' crossings themselves
+// (direct_deposit): forward single-scene runs, where the deposit code sits
+// outside the flight loop (kAux 0: exits recorded at the wave's refill), and
+// forward runs where registers are not the bound (FP64 and xoshiro check
+// runs, which thus take the same code path as the production kernels).
+// Gradient runs queue every crossing, their deposits carry scores; the other
+// FP32 Philox kernels queue them too (the deposit code would spill registers
+// the event loop needs: the flight-loop plane code of kAux >= 1, the batched
+// kernel's per-lane scene state).  The host reads the same function
+// (KParams::direct).
+constexpr bool deposits_direct(bool fp64, bool xo, bool adj, int aux, bool batch) {
+  return !adj && (fp64 || xo || (aux == 0 && !batch));
+}
+
+// The light sensors' process_hit at the crossing, inside the transport kernel
+// of forward runs: PlanarFluenceSensor (detector.cpp:780-841, lab x negated
+// at :812), FarFieldFluenceSensor (the forward field's Stokes vector in the
+// exit direction's bin) and ReflectanceSensor (the exit weight).  Same binning
+// and arithmetic as the sensor kernel's; the grid deposits are FP64 atomics
+// into HBM that the memory system completes while the kernel goes on
+// scattering, so these crossings take no queue record and no sensor-kernel
+// pass.  The reflectance weight and the hit counts, a few addresses every
+// photon adds to, are summed in the workgroup's LDS (single scenes; added to
+// HBM when the kernel ends) instead of by atomics on one HBM address each.
+template <class R, bool kXo, bool kBatch>
+__device__ __forceinline__ void direct_deposit(const KParams<R>& p, const SceneView<R>& v, const Photon<R, kXo>& ph,
+                                               int si, V3<R> hp, R arr) {
+  const SensorDev& sd = v.sens[si];
+  int64_t bin;
+  int t_idx;
+  if (!sensor_bin(sd, hp.x, hp.y, arr, ph.P.s, bin, t_idx)) return;
+  double* g = p.out + sd.out_off;
+  double refl = 0.0;
+  if (sd.kind == LMC_SENSOR_REFLECTANCE) {
+    refl = static_cast<double>(ph.w);
+  } else {
+    const R ws = M<R>::sqrt(ph.w);
+    const cx<R> fm = cscale(ph.Em, ws), fn = cscale(ph.En, ws);
+    const cx<R> Efx = cadd(cscale(fm, ph.P.m.x), cscale(fn, ph.P.n.x));
+    const cx<R> Efy = cadd(cscale(fm, ph.P.m.y), cscale(fn, ph.P.n.y));
+    const bool flu = sd.kind == LMC_SENSOR_PLANAR_FLUENCE;
+    const int64_t tplane = static_cast<int64_t>(sd.n0) * sd.n1;
+    const int64_t plane = flu ? tplane * sd.n2 : tplane;
+    const cx<R> Ex = flu ? cscale(Efx, R(-1)) : Efx;
+    const int64_t cell = static_cast<int64_t>(t_idx) * tplane + bin;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double val = static_cast<double>(stokes(q, Ex, Efy));
+      if (val != 0.0) unsafeAtomicAdd(g + q * plane + cell, val);
+    }
+  }
+  if constexpr (kBatch) {  // a batch's many sensors (FP64 and xoshiro check runs only)
+    if (refl != 0.0) unsafeAtomicAdd(g, refl);
+    atomicAdd(p.hits + si, 1ull);
+  } else {
+    if (refl != 0.0) __hip_atomic_fetch_add((lds_double*)(v.lrefl + si), refl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    atomicAdd(&v.lhits[si], 1u);
+  }
+}
+
 // A crossing of plane `pl` at hit point hp, corr (the overshoot from hp to
 // the segment's end) back along the flight: the overshoot-corrected paths
 // (detector.cpp:194-235); the sensors' filters decide acceptance and
-// absorption here, the deposits happen in k_crossings.  Returns true when an
-// absorbing sensor accepted the photon.
-template <class R, bool kXo, bool kAdj, int kAux>
+// absorption here, the light sensors of forward runs deposit here
+// (direct_deposit), the others' deposits happen in k_crossings from the
+// queued record.  Returns true when an absorbing sensor accepted the photon.
+template <class R, bool kXo, bool kAdj, int kAux, bool kBatch>
 __device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
                                              V3<R> hp, R corr, int cdir) {
   {
@@ -705,6 +768,12 @@ __device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneVie
     const SensorDev& sd = v.sens[first + k];
     if (!accepts(sd, hp, ph.P.s, cdir, ph.events)) continue;
     if (sd.absorb) killed = true;
+    if constexpr (deposits_direct(sizeof(R) == 8, kXo, kAdj, kAux, kBatch)) {
+      if (sd.direct) {
+        direct_deposit<R, kXo, kBatch>(p, v, ph, first + k, hp, arr);
+        continue;
+      }
+    }
     // only crossings that can land in a sensor's grid are queued
     if (-ph.P.s.z >= sd.q_cmin && hp.x >= sd.q_x0 && hp.x <= sd.q_x1 && hp.y >= sd.q_y0 && hp.y <= sd.q_y1 &&
         arr >= sd.q_t0 && arr <= sd.q_t1)
@@ -789,7 +858,7 @@ __device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneVie
 
 // A crossing of plane `pl` by the segment prev -> pos: hit point by linear
 // interpolation in z (detector.cpp:194-212), then plane_record.
-template <class R, bool kXo, bool kAdj, int kAux>
+template <class R, bool kXo, bool kAdj, int kAux, bool kBatch>
 __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R>& v, Photon<R, kXo>& ph, int pl,
                                           V3<R> prev, int cdir) {
   const R zp = plane_z<R>(v.planes[pl]);
@@ -797,7 +866,7 @@ __device__ __forceinline__ bool plane_hit(const KParams<R>& p, const SceneView<R
   const R t = qdiv(zp - prev.z, d.z);
   const V3<R> hp{prev.x + d.x * t, prev.y + d.y * t, prev.z + d.z * t};
   const V3<R> ov{hp.x - ph.pos.x, hp.y - ph.pos.y, hp.z - ph.pos.z};
-  return plane_record<R, kXo, kAdj, kAux>(p, v, ph, pl, hp, M<R>::sqrt(dot(ov, ov)), cdir);
+  return plane_record<R, kXo, kAdj, kAux, kBatch>(p, v, ph, pl, hp, M<R>::sqrt(dot(ov, ov)), cdir);
 }
 
 // SensorsGroup::record_hit (detector.cpp:178-270) for the segment prev->pos:
@@ -831,7 +900,7 @@ __device__ __forceinline__ bool record_hit(const KParams<R>& p, const SceneView<
   for (int pl = pl0; pl < pl1; ++pl) {
     const R zp = plane_z<R>(v.planes[pl]);
     if (zp >= zmin && zp <= zmax && !(from_iface && zp == z1))
-      killed |= plane_hit<R, kXo, kAdj, kAux>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
+      killed |= plane_hit<R, kXo, kAdj, kAux, kBatch>(p, v, ph, pl, prev, (z2 > z1) ? LMC_DIR_FORWARD : LMC_DIR_BACKWARD);
   }
   return killed;
 }
@@ -1406,6 +1475,9 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     // wave-uniform branch instead of ten selects in every iteration
     if (__ballot(evt == 1)) {
       if (evt == 1) {
+        // an empty volatile asm keeps this block a branch: speculated, it
+        // becomes ten selects in every event
+        asm volatile("");
         ph.r1 = ph.pos;
         ph.m1 = ph.P.m;
         ph.s1 = ph.P.s;
@@ -1535,7 +1607,8 @@ template <class R> struct LdsHdr {
   static constexpr size_t sensors = al16(species + sizeof(SpeciesDev<R>) * kMaxSpecies);
   static constexpr size_t planes = al16(sensors + sizeof(SensorDev) * kMaxSensors);
   static constexpr size_t lc = al16(planes + sizeof(PlaneDev) * kMaxSensors);
-  static constexpr size_t end = al16(lc + sizeof(LaunchConst<R>));
+  static constexpr size_t refl = al16(lc + sizeof(LaunchConst<R>));
+  static constexpr size_t end = al16(refl + sizeof(double) * kMaxSensors);
 };
 
 // The transport kernel.  kBatch: the photons of one launch belong to several
@@ -1569,6 +1642,7 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   }
   SceneView<R> v;
   v.lgrid = nullptr;
+  v.lrefl = nullptr;
   v.lhits = sHits;
   v.wq_base = sWB;
   v.wq_used = sWU;
@@ -1618,6 +1692,9 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     v.S = sS;
     v.sens = sSen;
     v.planes = sPl;
+    double* sRefl = reinterpret_cast<double*>(smem + H::refl);
+    for (int i = tid; i < kMaxSensors; i += nt) sRefl[i] = 0.0;
+    v.lrefl = sRefl;
     q = smem + H::end;
   }
   if constexpr (kLdsTables && !kBatch) {
@@ -1731,6 +1808,15 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
   }
   __syncthreads();
   if (tid == 0) atomicAdd(&p.stats[3], static_cast<unsigned long long>(sHits[kMaxSensors]));
+  if constexpr (!kBatch && deposits_direct(sizeof(R) == 8, kXo, kAdj, kAux, false)) {
+    // the workgroup's hit counts and reflectance sums of the direct sensors
+    for (int si = tid; si < p.n_sensors; si += nt) {
+      const SensorDev& sd = v.sens[si];
+      if (!sd.direct) continue;
+      if (sHits[si]) atomicAdd(&p.hits[si], static_cast<unsigned long long>(sHits[si]));
+      if (v.lrefl[si] != 0.0) unsafeAtomicAdd(p.out + sd.out_off, v.lrefl[si]);
+    }
+  }
   if (lane == 0) atomicAdd(&p.stats[1], static_cast<unsigned long long>(w_ev));
   if constexpr (kEst) {  // estimator hits and the LDS grid copies go to HBM
     for (int si = 0; si < p.n_sensors; ++si) {

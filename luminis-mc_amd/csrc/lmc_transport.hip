@@ -327,7 +327,6 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.mfp = static_cast<R>(d.mfp);
     o.mu_t = static_cast<R>(d.mu_t);
     o.albedo = static_cast<R>(d.mu_t > 0 ? d.mu_s / d.mu_t : 1.0);
-    o.n = static_cast<R>(d.n);
     o.scene = s->layer_scene[i];
     o.n64 = d.n;
     o.mu_s = static_cast<R>(d.mu_s);
@@ -335,9 +334,9 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.sp_first = d.species_first;
     o.sp_count = d.species_count;
     o.sel_off = static_cast<int32_t>(sel.size());
-    o.nbr = (li + 1 < sc.n_layers ? 1 : 0) | (li > 0 ? 2 : 0);
+    o.nbr = static_cast<int16_t>((li + 1 < sc.n_layers ? 1 : 0) | (li > 0 ? 2 : 0));
     o.pl_first = sc.plane_first;
-    o.pl_count = sc.plane_count;
+    o.pl_count = static_cast<int16_t>(sc.plane_count);
     o.z_top = static_cast<R>(s->layers[sc.layer_base + sc.n_layers - 1].z_max);  // +inf when semi-infinite
     if (d.species_count > 1)
       for (int j = 0; j < d.species_count; ++j) sel.push_back(static_cast<R>(s->tables[d.select_offset + j]));
@@ -587,8 +586,8 @@ bool bin_eligible(const SensorDev& o) {
 
 // Binned HBM deposits one accepted crossing makes into a sensor's grids (only
 // sensors marked for binned deposits have any).
-int hbm_deposits(const SensorDev& sd) {
-  if (sd.lds || sd.est || !sd.bin) return 0;
+int hbm_deposits(const SensorDev& sd, bool direct_run) {
+  if (sd.lds || sd.est || !(direct_run ? sd.bin_d : sd.bin)) return 0;
   switch (sd.kind) {
     case LMC_SENSOR_FARFIELD_CBS: return 8;
     case LMC_SENSOR_PLANAR_FLUENCE:
@@ -788,6 +787,8 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   // only when it leaves the sample
   const int aux_mode =
       abs_run ? 2 : ((has_stats || rec_run || est || s->flight_planes || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
+  // the light sensors' crossings: deposited by the transport kernel or queued
+  p.direct = deposits_direct(sizeof(R) == 8, kXo, kAdj, aux_mode, batch) ? 1 : 0;
   auto pick = [&](bool wide, bool lds_t) -> TransportFn<R> {
     return transport_kernel<R, kXo, kAdj>(aux_mode, est, wide, lds_t, batch, track);
   };
@@ -875,7 +876,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   int max_dep = 0;
   for (const PlaneDev& P : s->planes_host) {
     int d = 0;
-    for (int k = 0; k < P.count; ++k) d += hbm_deposits(s->sensors_host[P.first + k]);
+    for (int k = 0; k < P.count; ++k) d += hbm_deposits(s->sensors_host[P.first + k], p.direct != 0);
     max_dep = std::max(max_dep, d);
   }
   const int64_t bin_tiles = (s->n_out + (int64_t(1) << kBinShift) - 1) >> kBinShift;
@@ -1250,6 +1251,12 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
         o.n1 = o.n2 = 0;
       }
       o.est = sd.estimator ? 1 : 0;
+      // forward runs deposit the light sensors' crossings in the transport
+      // kernel (no queue record); gradient runs still queue them
+      o.direct = (!o.est && (sd.kind == LMC_SENSOR_PLANAR_FLUENCE || sd.kind == LMC_SENSOR_FARFIELD_FLUENCE ||
+                             sd.kind == LMC_SENSOR_REFLECTANCE))
+                     ? 1
+                     : 0;
       o.est_lds = -1;
       if (o.est) {
         // no crossings reach an estimator-mode sensor (detector.cpp:246-253)
@@ -1323,10 +1330,17 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
 }
 
 int finish_set(lmc_scene* s) {
-  bool any_bin = false;
-  for (const SensorDev& o : s->sensors_host) any_bin = any_bin || o.bin;
-  if (any_bin)
-    for (SensorDev& o : s->sensors_host) o.bin = bin_eligible(o) ? 1 : 0;
+  // two plans: runs that queue every crossing (bin), and runs whose transport
+  // kernel deposits the direct sensors (bin_d: those grids are no trigger)
+  bool any_bin = false, any_bin_d = false;
+  for (const SensorDev& o : s->sensors_host) {
+    any_bin = any_bin || o.bin;
+    any_bin_d = any_bin_d || (o.bin && !o.direct);
+  }
+  for (SensorDev& o : s->sensors_host) {
+    o.bin_d = (any_bin_d && bin_eligible(o) && !o.direct) ? 1 : 0;
+    if (any_bin) o.bin = bin_eligible(o) ? 1 : 0;
+  }
   int rc = s->sensors_dev.upload(s->sensors_host.data(), s->sensors_host.size());
   if (!rc) rc = s->planes_dev.upload(s->planes_host.data(), s->planes_host.size());
   if (!rc) rc = s->out.alloc(std::max<int64_t>(1, s->n_out));
