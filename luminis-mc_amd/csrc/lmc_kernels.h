@@ -34,10 +34,12 @@ template <class R> struct LaunchConst {
 // per-event scalars apart from the per-photon refill keys (a spilled group is
 // reloaded whole: 8 lane reads where one value is needed).
 template <class R> struct KParams {
-  // the phase and amplitude tables in one blob: theta CDFs (R), their guide
-  // tables (u16), S1/S2 nodes (STab<R>) and species-selection CDFs (R), each
-  // section 32-byte aligned; species and layers hold offsets from the blob's
-  // start in units of the element type, so one base address serves all four
+  // the phase and amplitude tables in one blob: theta CDFs (R), S1/S2 nodes
+  // (STab<R>), species-selection CDFs (R) and the CDFs' guide tables (u16),
+  // each section 32-byte aligned; species and layers hold offsets from the
+  // blob's start in units of the element type, so one base address serves all
+  // four.  The first tab_bytes (all but the guides) are what a launch with
+  // LDS-resident tables copies to LDS.
   const unsigned char* tab;
   double* out;
   unsigned long long* hits;
@@ -1701,7 +1703,9 @@ __device__ __forceinline__ void transport_body(const KParams<R>& p) {
     unsigned char* tb = carve(p.tab_bytes);
     copy(tb, p.tab, p.tab_bytes);
     v.cdf = reinterpret_cast<const R*>(tb);
-    v.guide = reinterpret_cast<const uint16_t*>(tb);
+    // the guide tables stay in global memory (cache resident): the LDS copy
+    // ends where they begin
+    v.guide = reinterpret_cast<const uint16_t*>(p.tab);
     v.stab = reinterpret_cast<const STab<R>*>(tb);
     v.sel = reinterpret_cast<const R*>(tb);
   } else {

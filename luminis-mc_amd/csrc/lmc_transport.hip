@@ -19,6 +19,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -196,6 +197,8 @@ struct lmc_batch {
 namespace {
 
 constexpr int kGuideN = 4096;
+constexpr int kGuideFineN = 65536;
+constexpr int kFineGuideMaxCdfs = 8;
 constexpr int kMaxBatchScenes = 4096;
 
 // compute_I_norm (detector.cpp:105-118): integral of (|S1|^2 + |S2|^2) sin(theta)
@@ -231,11 +234,22 @@ double i_norm_of(const lmc_species_desc& d, const double* tables) {
 
 template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if (T.ready) return LMC_OK;
-  // guide entries per CDF: a batch reads its tables through the caches, where
-  // a 16x finer guide (bracket of about one node, fewer dependent loads per
-  // theta sample) ran 6 % faster; a single scene keeps the small guide so its
-  // tables fit in LDS
-  const int gN = s->scenes.size() > 1 ? 16 * kGuideN : kGuideN;
+  // Guide entries per CDF.  The guides are read from global memory (cache
+  // resident) in every launch, also when the CDFs and S tables sit in LDS.  A
+  // 65536-entry guide brackets about one node, so a theta sample takes fewer
+  // dependent loads and the wave's search loop runs fewer rounds than with
+  // 4096 entries (the wave waits for its widest bracket: about 2 rounds instead
+  // of 4-6 on 10^4-node Mie and RDG tables): a batch ran 6 % faster with it.
+  // A single scene with many distinct CDFs keeps 4096 entries: the fine guides
+  // of a 32-species mixture (4 MB) no longer stay in L2 (6 % slower there).
+  int n_cdfs = 0;
+  {
+    std::set<int64_t> offs;
+    for (const lmc_species_desc& d : s->species)
+      if (d.phase_kind == LMC_PHASE_TABLE_THETA || d.phase_kind == LMC_PHASE_TABLE_COS) offs.insert(d.cdf_offset);
+    n_cdfs = static_cast<int>(offs.size());
+  }
+  const int gN = (s->scenes.size() > 1 || n_cdfs <= kFineGuideMaxCdfs) ? kGuideFineN : kGuideN;
   std::vector<SpeciesDev<R>> sp(s->species.size());
   std::vector<R> cdf;
   std::vector<uint16_t> guide;
@@ -390,13 +404,14 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     if (eg.empty()) eg.push_back(R(0));
     if ((rc = T.aux.upload(eg.data(), eg.size()))) return rc;
   }
-  // one blob, sections 32-byte aligned (an STab<double> is 32 bytes); the
-  // species' and layers' offsets move by their section's start
+  // one blob, sections 32-byte aligned (an STab<double> is 32 bytes), the
+  // guides last (never copied to LDS); the species' and layers' offsets move
+  // by their section's start
   auto al32 = [](size_t b) { return (b + 31) & ~static_cast<size_t>(31); };
-  const size_t off_g = al32(sizeof(R) * cdf.size());
-  const size_t off_s = off_g + al32(sizeof(uint16_t) * guide.size() + 4);
+  const size_t off_s = al32(sizeof(R) * cdf.size());
   const size_t off_l = off_s + al32(sizeof(STab<R>) * stab.size());
-  const size_t total = off_l + al32(sizeof(R) * sel.size() + 4);
+  const size_t off_g = off_l + al32(sizeof(R) * sel.size() + 4);
+  const size_t total = off_g + al32(sizeof(uint16_t) * guide.size() + 4);
   if (total > 0x7FFFFFF0ull) return fail(LMC_ERR_INVALID, "tables too large");
   for (SpeciesDev<R>& o : sp) {
     o.guide_off += static_cast<int32_t>(off_g / sizeof(uint16_t));
@@ -411,7 +426,7 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   if (!stab.empty()) std::memcpy(blob.data() + off_s, stab.data(), sizeof(STab<R>) * stab.size());
   if (!sel.empty()) std::memcpy(blob.data() + off_l, sel.data(), sizeof(R) * sel.size());
   if ((rc = T.tab.upload(blob.data(), blob.size()))) return rc;
-  T.tab_bytes = total;
+  T.tab_bytes = off_g;  // the LDS part: CDFs, S tables, selection CDFs
   T.ready = true;
   return LMC_OK;
 }
