@@ -198,7 +198,7 @@ namespace {
 
 constexpr int kGuideN = 4096;
 constexpr int kGuideFineN = 65536;
-constexpr int kFineGuideMaxCdfs = 8;
+constexpr int64_t kGuideBudget = int64_t(1) << 20;  // bytes of guide tables a single scene keeps
 constexpr int kMaxBatchScenes = 4096;
 
 // compute_I_norm (detector.cpp:105-118): integral of (|S1|^2 + |S2|^2) sin(theta)
@@ -239,17 +239,20 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   // 65536-entry guide brackets about one node, so a theta sample takes fewer
   // dependent loads and the wave's search loop runs fewer rounds than with
   // 4096 entries (the wave waits for its widest bracket: about 2 rounds instead
-  // of 4-6 on 10^4-node Mie and RDG tables): a batch ran 6 % faster with it.
-  // A single scene with many distinct CDFs keeps 4096 entries: the fine guides
-  // of a 32-species mixture (4 MB) no longer stay in L2 (6 % slower there).
-  int n_cdfs = 0;
-  {
+  // of 4-6 on 10^4-node Mie and RDG tables): a batch ran 6 % faster with it,
+  // and a batch keeps only a few scenes' guides in flight.  A single scene's
+  // guides all stay in use, so they get the largest power of two entries
+  // (4096 to 65536) whose guides fit kGuideBudget bytes in all: 65536 for up
+  // to 8 distinct CDFs, 16384 for a 32-species mixture (whose 65536-entry
+  // guides, 4 MB, no longer stayed in L2: 6 % slower than 4096).
+  int gN = kGuideFineN;
+  if (s->scenes.size() == 1) {
     std::set<int64_t> offs;
     for (const lmc_species_desc& d : s->species)
       if (d.phase_kind == LMC_PHASE_TABLE_THETA || d.phase_kind == LMC_PHASE_TABLE_COS) offs.insert(d.cdf_offset);
-    n_cdfs = static_cast<int>(offs.size());
+    const int64_t n_cdfs = std::max<int64_t>(1, static_cast<int64_t>(offs.size()));
+    while (gN > kGuideN && n_cdfs * gN * static_cast<int64_t>(sizeof(uint16_t)) > kGuideBudget) gN >>= 1;
   }
-  const int gN = (s->scenes.size() > 1 || n_cdfs <= kFineGuideMaxCdfs) ? kGuideFineN : kGuideN;
   std::vector<SpeciesDev<R>> sp(s->species.size());
   std::vector<R> cdf;
   std::vector<uint16_t> guide;
