@@ -48,9 +48,11 @@ PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no pac
 # FETCH_SIZE / WRITE_SIZE over one 4e6-photon launch, scaled by what the same
 # counters read for a device-to-device copy of known size (`bench.py
 # --copy-probe 256`): fetch x2.00, write x1.00.  Table reads are cache
-# resident; the writes are the crossing records handed to the sensor kernel.
-HBM_FETCH_B_PER_PHOTON = 691.8 * 1024 * 2.00 / 4e6
-HBM_WRITE_B_PER_PHOTON = 6.519e5 * 1024 * 1.00 / 4e6
+# resident; the writes are the light sensors' FP64 deposits (planar fluence,
+# far-field fluence: 6.4 per photon, one 32-byte sector each) and the few CBS
+# crossing records handed to the sensor kernel.
+HBM_FETCH_B_PER_PHOTON = 1652.7 * 1024 * 2.00 / 4e6
+HBM_WRITE_B_PER_PHOTON = 8.0713e5 * 1024 * 1.00 / 4e6
 
 
 def log(msg: str) -> None:
