@@ -4,7 +4,9 @@ Same public names as the reference package's transport surface
 (python/luminis_mc/__init__.py:1-76): build a Sample, a Laser and a
 SensorsGroup, set up a SimConfig, call run_simulation_parallel(config) and
 post-process the sensors with postprocess_*.  The transport runs as HIP
-kernels on the GPU through libluminis_hip.so.
+kernels on the GPU through libluminis_hip.so.  Results persist to the
+reference's HDF5 layout through Experiment / ResultsLoader (:mod:`manager`,
+the HDF5 C library through ctypes).
 """
 from ._lib import lib as _load_library
 from .linalg import (CMatrix, CVec2, Matrix, Rng, Vec2, Vec3, calculate_rotation_angle, cross, dot, matcmul,
@@ -28,6 +30,13 @@ from .absorption import Absorption
 from .photon import LogLevel, Photon, gaussian_distribution, get_log_level, set_log_level, uniform_distribution
 from .tuning import autotune
 from .adjoint import DifferentiableTransport, crn_central_difference
+from .manager import (Experiment, ResultsLoader, capture_params, derived_quantities, derived_quantities_mixture,
+                      on_progress)
+from .records import (AbsorptionResult, FarFieldCBSProcessedResult, FarFieldCBSResult, LaserParams, LayerParams,
+                      MediumParams, PhotonRecordsResult, PlanarFieldProcessedResult, PlanarFieldResult,
+                      PlanarFluenceProcessedResult, PlanarFluenceResult, RunParams, SensorMeta, SimParams,
+                      StatisticsResult)
+from .schema import SENSOR_SCHEMAS, SensorSchema
 
 _load_library()
 

@@ -299,8 +299,10 @@ def test_lognormal_mixture_weights_and_total_scattering():
 
 def test_reference_public_names_are_exported():
     """Every transport-surface name of the reference package
-    (python/luminis_mc/__init__.py:1-76) imports from luminis_mc; the MCMC
-    mean-free-path helpers and the HDF5 manager are not on the transport path."""
+    (python/luminis_mc/__init__.py:1-76) and its persistence names (:77-89,
+    the HDF5 manager, typed records and schema registry) import from
+    luminis_mc; the MCMC mean-free-path helpers and the sweep manifest are not
+    on the transport path."""
     names = """Rng Vec3 Vec2 CVec2 Matrix CMatrix dot cross norm matcmul matcmulscalar calculate_rotation_angle
     PhaseFunction UniformPhaseFunction RayleighPhaseFunction HenyeyGreensteinPhaseFunction RayleighDebyePhaseFunction
     RayleighDebyeEMCPhaseFunction DrainePhaseFunction MiePhaseFunction form_factor Photon PhotonRecord LaserSource
@@ -309,7 +311,11 @@ def test_reference_public_names_are_exported():
     StokesRadialProcessed FarFieldCBSProcessed FarFieldCBSRadialProcessed PlanarFluenceProcessed
     PlanarFieldProcessed postprocess_farfield_cbs postprocess_planar_fluence postprocess_planar_field
     ScatteringMedium RGDMedium MieMedium Layer SampleLayer MixtureLayer Sample Absorption SimConfig
-    run_simulation_parallel LogLevel set_log_level""".split()
+    run_simulation_parallel LogLevel set_log_level Experiment ResultsLoader capture_params derived_quantities
+    derived_quantities_mixture SimParams RunParams LaserParams MediumParams LayerParams SensorMeta FarFieldCBSResult
+    PlanarFluenceResult PlanarFieldResult StatisticsResult PhotonRecordsResult AbsorptionResult
+    FarFieldCBSProcessedResult PlanarFluenceProcessedResult PlanarFieldProcessedResult SensorSchema
+    SENSOR_SCHEMAS""".split()
     missing = [n for n in names if not hasattr(lmc, n)]
     assert not missing, missing
     assert [int(lmc.LogLevel.debug), int(lmc.LogLevel.info), int(lmc.LogLevel.warn), int(lmc.LogLevel.error),
