@@ -110,11 +110,13 @@ template <class R> struct KParams {
 //         before the last one (m, s), raw interior product T (4 complex),
 //         r_n - r_1                             (photons with >= 2 events at
 //                                               a plane holding a CBS sensor)
-//   Q10-Q12 adjoint scores (adjoint runs only)
+//   Q10-Q11 PhotonRecordSensor extras: r_1, launch time, r_n, path at the
+//           crossing                            (planes holding one)
+//   Q12-Q14 adjoint scores (adjoint runs only)
 // The plane, sensor mask, event count and species pair go to int arrays, the
 // FP64 optical path to a double array (both indexed by slot).
-constexpr int kRecQ = 13;
-constexpr int kRecQLight = 5, kRecQCbs = 10;
+constexpr int kRecQ = 15;
+constexpr int kRecQLight = 5, kRecQCbs = 10, kRecQScore = 12;
 
 template <class R> __device__ __forceinline__ void put4(R* f, int q, R a, R b, R c, R d) {
   if constexpr (sizeof(R) == 4) {
@@ -847,7 +849,8 @@ __device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneVie
     R score[3 * kMaxGradLayers];
     adj_scores(p, v, ph, corr, score);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) put4(f, 10 + q, score[4 * q], score[4 * q + 1], score[4 * q + 2], score[4 * q + 3]);
+    for (int q = 0; q < 3; ++q)
+      put4(f, kRecQScore + q, score[4 * q], score[4 * q + 1], score[4 * q + 2], score[4 * q + 3]);
   }
   int32_t* iq = p.qi + slot;
   iq[I_PLANE * c] = pl;
@@ -1624,7 +1627,6 @@ template <class R> struct LdsHdr {
 // LDS for tables that two workgroups could not hold).
 template <class R, bool kXo, bool kTrack, bool kLdsTables, bool kAdj, bool kBatch, bool kEst = false, int kAux = 1>
 __device__ __forceinline__ void transport_body(const KParams<R>& p) {
-  static_assert(kAux != 2 || !kAdj, "the Absorption grid is recorded by forward runs");
   extern __shared__ __align__(16) unsigned char smem[];
   using H = LdsHdr<R>;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1934,7 +1936,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
     if constexpr (kAdj) {
 #pragma unroll
       for (int qq = 0; qq < 3; ++qq) {
-        const Quad<R> sq = get4(f, 10 + qq);
+        const Quad<R> sq = get4(f, kRecQScore + qq);
         score[4 * qq] = sq.a;
         score[4 * qq + 1] = sq.b;
         score[4 * qq + 2] = sq.c;

@@ -788,9 +788,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   if (est && !r->track_reverse_paths)
     return fail(LMC_ERR_INVALID, "the CBS estimator needs track_reverse_paths (reverse path of every forced scatter)");
   const bool abs_run = p.abs_on != 0;
-  if (abs_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "the Absorption grid is recorded by forward runs");
   const bool rec_run = s->rec_per_crossing > 0;
-  if (rec_run && kAdj) return fail(LMC_ERR_UNSUPPORTED, "PhotonRecordSensor records are kept by forward runs");
   if (rec_run && scratch)
     return fail(LMC_ERR_UNSUPPORTED, "no-alloc runs keep no PhotonRecordSensor list (it is sized per launch): "
                                      "use lmc_scene_run");

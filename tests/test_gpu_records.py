@@ -7,8 +7,9 @@
 * FP32 production path: counts, weight sums against a reflectance sensor,
   frames, the rebuilt CBS grid, and a run long enough to span several
   transport launches (record lists gathered per launch).
-* run_simulation_parallel fills recorded_photons; gradient runs refuse record
-  sensors (batched runs keep them per scene: test_gpu_batch_extras.py).
+* run_simulation_parallel fills recorded_photons; gradient runs keep the same
+  records as forward runs (batched runs keep them per scene:
+  test_gpu_batch_extras.py).
 """
 import math
 
@@ -139,9 +140,23 @@ def test_run_simulation_parallel_fills_recorded_photons(require_gpu):
     del b
 
 
-def test_record_sensor_refusals(require_gpu):
+def test_records_in_adjoint_runs_equal_forward(require_gpu):
+    """A gradient run keeps the PhotonRecordSensor's records as a forward run
+    does: on FP64 xoshiro streams the same crossings, field for field (their
+    adjoint scores travel in separate quads of the crossing record)."""
     b = _record_scene()
     sc = GpuScene(None, None, None, build=b)
-    with pytest.raises(NotImplementedError):
-        sc.run_adjoint(_rd("fp32", "philox", 1000, 5))
+    n, seed = 6000, 19
+    grad, _ = sc.run_adjoint(_rd("fp64", "xoshiro", n, seed))
+    a_out, a_hits = sc.read()
+    a = _sorted(sc.records())
+    sc.zero()
+    sc.run(_rd("fp64", "xoshiro", n, seed))
+    f_out, f_hits = sc.read()
+    f = _sorted(sc.records())
+    print("records adjoint/forward", len(a), len(f), "hits", list(a_hits), list(f_hits))
+    assert len(a) == len(f) == int(f_hits[1]) > 0 and list(a_hits) == list(f_hits)
+    assert np.abs(a - f).max() <= 1e-12 * max(1.0, np.abs(f).max())
+    assert np.abs(a_out - f_out).max() <= 1e-12 * np.abs(f_out).max()
+    assert grad.any()
     sc.close()
