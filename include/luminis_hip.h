@@ -316,7 +316,7 @@ int lmc_scene_zero(lmc_scene* s);
 /* PhotonRecordSensor records gathered since creation or the last zero, in
  * crossing-queue order (not photon order): copies min(max_records, count)
  * records of LMC_RECORD_FIELDS doubles into `out` (may be NULL to count
- * only) and sets *n_records to the count.  Forward runs only.
+ * only) and sets *n_records to the count.  Forward and gradient runs alike.
  * Replaces: PhotonRecordSensor::recorded_photons (detector.hpp:280-298) and
  * its merge_from (detector.cpp:455-461). */
 int lmc_scene_records(lmc_scene* s, double* out, int64_t max_records, int64_t* n_records);
