@@ -15,6 +15,9 @@ constexpr int kMaxLayers = 8;
 constexpr int kMaxSensors = 8;
 constexpr int kMaxSpecies = 64;
 constexpr int kMaxGradLayers = 4;
+// guide entries of a mixture layer's species-selection CDF (u16, kSelGuide + 1
+// after the CDF's kMaxSpecies-at-most values)
+constexpr int kSelGuide = 256;
 
 // ------------------------------------------------------------ math ------
 template <class R> struct M;

@@ -1347,8 +1347,17 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
         u = static_cast<R>(ph.xo.s.uniform());
       else
         u = aux_uniform<R>(ph);
+      // lower_bound of u in the selection CDF (std::lower_bound in the
+      // reference), narrowed by the layer's guide g[j] = lower_bound(j / G)
+      // stored after the CDF (g[G] = K): the bracket [g[j], g[j+1]] holds the
+      // answer, usually one or two nodes, so a 32-species mixture takes one
+      // search round instead of five dependent loads
       const R* c = v.sel + Lv.sel_off;
-      int lo = 0, hi = Lv.sp_count;
+      const int K = Lv.sp_count;
+      const uint16_t* gd = reinterpret_cast<const uint16_t*>(c + K);
+      int j = static_cast<int>(u * static_cast<R>(kSelGuide));
+      j = j < 0 ? 0 : (j >= kSelGuide ? kSelGuide - 1 : j);
+      int lo = gd[j], hi = gd[j + 1];
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (c[mid] < u)
@@ -1356,7 +1365,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
         else
           hi = mid;
       }
-      sp += lo < Lv.sp_count ? lo : Lv.sp_count - 1;
+      sp += lo < K ? lo : K - 1;
     }
     if constexpr (kPart == 1) {
       ph.cur_sp = sp;

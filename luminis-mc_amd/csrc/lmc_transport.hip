@@ -355,8 +355,25 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
     o.pl_first = sc.plane_first;
     o.pl_count = static_cast<int16_t>(sc.plane_count);
     o.z_top = static_cast<R>(s->layers[sc.layer_base + sc.n_layers - 1].z_max);  // +inf when semi-infinite
-    if (d.species_count > 1)
-      for (int j = 0; j < d.species_count; ++j) sel.push_back(static_cast<R>(s->tables[d.select_offset + j]));
+    if (d.species_count > 1) {
+      // the selection CDF in R, then its guide: g[j] = lower_bound(cdf, j / G)
+      // on the converted values (the kernel compares in R), g[G] = K, as u16
+      // packed into R slots
+      const int K = d.species_count;
+      std::vector<R> cr(K);
+      for (int j = 0; j < K; ++j) cr[j] = static_cast<R>(s->tables[d.select_offset + j]);
+      sel.insert(sel.end(), cr.begin(), cr.end());
+      std::vector<uint16_t> g(kSelGuide + 1);
+      for (int j = 0; j < kSelGuide; ++j) {
+        const R thr = static_cast<R>(static_cast<double>(j) / kSelGuide);
+        g[j] = static_cast<uint16_t>(std::lower_bound(cr.begin(), cr.end(), thr) - cr.begin());
+      }
+      g[kSelGuide] = static_cast<uint16_t>(K);
+      const size_t slots = (sizeof(uint16_t) * g.size() + sizeof(R) - 1) / sizeof(R);
+      const size_t at = sel.size();
+      sel.resize(at + slots, R(0));
+      std::memcpy(sel.data() + at, g.data(), sizeof(uint16_t) * g.size());
+    }
   }
   std::vector<LaunchConst<R>> lc(s->scenes.size());
   for (size_t k = 0; k < s->scenes.size(); ++k) {
