@@ -181,7 +181,8 @@ template <class R, bool kXo> struct Photon {
   // w_th, w_ph, w_az), 2 when that segment also started on an interface
   int32_t pend;
   // CBS history
-  V3<R> m1, s1, mq, sq;  // frame after the first scatter, frame before the last one
+  V3<R> m1, s1;          // frame after the first scatter
+  R qc, qs;              // cos and sin of the last scattering angle: the frame before the last scatter is A^T P
   V3<R> r1, rn;
   cx<R> T[4];            // raw interior product J_{n-1}...J_2 (Frobenius-normalised)
   cx<R> b2, b1;          // last event's J_n = diag(b2, b1) R(bc, bs), pending
@@ -833,9 +834,17 @@ __device__ __forceinline__ bool plane_record(const KParams<R>& p, const SceneVie
   put4(f, 4, ph.P.s.z, ph.m1.x, ph.m1.y, ph.m1.z);
   p.qd[slot] = opl_hit;
   if (ph.events >= 2 && (v.planes[pl].flags & kPlaneCbs)) {
-    put4(f, 5, ph.s1.x, ph.s1.y, ph.s1.z, ph.mq.x);
-    put4(f, 6, ph.mq.y, ph.mq.z, ph.sq.x, ph.sq.y);
-    put4(f, 7, ph.sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r);
+    // the frame before the last scatter, Po = A(theta, phi)^T P with the
+    // last event's angles (A is orthogonal)
+    const Frame<R>& P = ph.P;
+    const R c = ph.qc, s = ph.qs, cp = ph.bc, sn = ph.bs;
+    const R cc = c * cp, sc = s * cp;
+    const V3<R> mq{cc * P.m.x - sn * P.n.x + sc * P.s.x, cc * P.m.y - sn * P.n.y + sc * P.s.y,
+                   cc * P.m.z - sn * P.n.z + sc * P.s.z};
+    const V3<R> sq{c * P.s.x - s * P.m.x, c * P.s.y - s * P.m.y, c * P.s.z - s * P.m.z};
+    put4(f, 5, ph.s1.x, ph.s1.y, ph.s1.z, mq.x);
+    put4(f, 6, mq.y, mq.z, sq.x, sq.y);
+    put4(f, 7, sq.z, ph.T[0].r, ph.T[0].i, ph.T[1].r);
     put4(f, 8, ph.T[1].i, ph.T[2].r, ph.T[2].i, ph.T[3].r);
     put4(f, 9, ph.T[3].i, ph.rn.x - ph.r1.x, ph.rn.y - ph.r1.y, ph.rn.z - ph.r1.z);
   }
@@ -1213,8 +1222,8 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
   ph.pend = 0;
   ph.m1 = ph.P.m;
   ph.s1 = ph.P.s;
-  ph.mq = ph.P.m;
-  ph.sq = ph.P.s;
+  ph.qc = R(1);
+  ph.qs = R(0);
   ph.r1 = ph.pos;
   ph.rn = ph.pos;
   ph.T[0] = {R(1), R(0)};
@@ -1374,6 +1383,36 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
   }
   const LayerDev<R>& Ld = v.L[ph.layer];
   const SpeciesDev<R>& S = v.S[sp];
+  if constexpr (kTrack) {
+    // the pending J_{n-1} of the last event is committed before this event's
+    // sampling (its factors are no longer needed once the new ones exist)
+    const int evt = ph.events + 1;
+    if (evt >= 3) {
+      // commit the pending J_{n-1} = diag(b2, b1) [[bc, bs], [-bs, bc]]:  T <- J T, Frobenius-normalised.
+      // Only T's direction is used (the reverse field is normalised), so the
+      // FP32 Philox path normalises every other wave iteration (renorm, a
+      // wave-uniform flag): between normalisations |T| stays within one
+      // factor max|S1|,|S2| of one.  The check modes normalise every event.
+      const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
+      const cx<R> r1 = cadd(cscale(ph.T[1], ph.bc), cscale(ph.T[3], ph.bs));
+      const cx<R> r2 = cadd(cscale(ph.T[0], -ph.bs), cscale(ph.T[2], ph.bc));
+      const cx<R> r3 = cadd(cscale(ph.T[1], -ph.bs), cscale(ph.T[3], ph.bc));
+      const cx<R> t0 = cmul(ph.b2, r0), t1 = cmul(ph.b2, r1), t2 = cmul(ph.b1, r2), t3 = cmul(ph.b1, r3);
+      if (kXo || sizeof(R) == 8 || renorm) {
+        const R fr2 = cnorm(t0) + cnorm(t1) + cnorm(t2) + cnorm(t3);
+        const R inv = fr2 > R(0) ? M<R>::rsqrt(fr2) : R(1);
+        ph.T[0] = cscale(t0, inv);
+        ph.T[1] = cscale(t1, inv);
+        ph.T[2] = cscale(t2, inv);
+        ph.T[3] = cscale(t3, inv);
+      } else {
+        ph.T[0] = t0;
+        ph.T[1] = t1;
+        ph.T[2] = t2;
+        ph.T[3] = t3;
+      }
+    }
+  }
   // theta (medium.cpp:51-58 -> phase.cpp)
   R uth;
   if constexpr (kXo)
@@ -1500,33 +1539,8 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     }
     ph.rn = ph.pos;
     ph.sp = (ph.sp & 0xFFFF0000) | sp;
-    ph.mq = Po.m;
-    ph.sq = Po.s;
-    if (evt >= 3) {
-      // commit the pending J_{n-1} = diag(b2, b1) [[bc, bs], [-bs, bc]]:  T <- J T, Frobenius-normalised.
-      // Only T's direction is used (the reverse field is normalised), so the
-      // FP32 Philox path normalises every other wave iteration (renorm, a
-      // wave-uniform flag): between normalisations |T| stays within one
-      // factor max|S1|,|S2| of one.  The check modes normalise every event.
-      const cx<R> r0 = cadd(cscale(ph.T[0], ph.bc), cscale(ph.T[2], ph.bs));
-      const cx<R> r1 = cadd(cscale(ph.T[1], ph.bc), cscale(ph.T[3], ph.bs));
-      const cx<R> r2 = cadd(cscale(ph.T[0], -ph.bs), cscale(ph.T[2], ph.bc));
-      const cx<R> r3 = cadd(cscale(ph.T[1], -ph.bs), cscale(ph.T[3], ph.bc));
-      const cx<R> t0 = cmul(ph.b2, r0), t1 = cmul(ph.b2, r1), t2 = cmul(ph.b1, r2), t3 = cmul(ph.b1, r3);
-      if (kXo || sizeof(R) == 8 || renorm) {
-        const R fr2 = cnorm(t0) + cnorm(t1) + cnorm(t2) + cnorm(t3);
-        const R inv = fr2 > R(0) ? M<R>::rsqrt(fr2) : R(1);
-        ph.T[0] = cscale(t0, inv);
-        ph.T[1] = cscale(t1, inv);
-        ph.T[2] = cscale(t2, inv);
-        ph.T[3] = cscale(t3, inv);
-      } else {
-        ph.T[0] = t0;
-        ph.T[1] = t1;
-        ph.T[2] = t2;
-        ph.T[3] = t3;
-      }
-    }
+    ph.qc = ct;
+    ph.qs = st;
     ph.b2 = s2;
     ph.b1 = s1;
     ph.bc = cph;
