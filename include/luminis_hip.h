@@ -386,6 +386,11 @@ int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
  * 0xFFFFFFFE, 0). */
 int lmc_probe_stream(uint32_t k0, uint32_t k1, uint64_t photon, int32_t n, uint32_t* out);
 
+/* FP32 path phase on device: out[2i], out[2i+1] = sin x[i], cos x[i] as the
+ * FP32 CBS reverse-path factor exp(i k0 (s_out + s_in).(r_n - r_1)) forms
+ * them (detector.cpp:1231-1310), for phases of any size. */
+int lmc_probe_phase(int32_t n, const float* x, float* out);
+
 /* Azimuth sampler of the production path on device: for each i, the (cos phi,
  * sin phi, F(phi)) that the transport draws from the two 32-bit words
  * words[2i] (component and radius) and words[2i+1] (angle, half period) for

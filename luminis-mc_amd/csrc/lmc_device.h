@@ -50,7 +50,15 @@ template <> struct M<float> {
     *s = __builtin_amdgcn_sinf(u);
     *c = __builtin_amdgcn_cosf(u);
   }
+  // angles within a few revolutions (scattering angles)
   static __device__ __forceinline__ void sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
+  // path phases of any size: reduced to [0, 1) revolutions first (v_fract is
+  // exact), since v_sin / v_cos take revolutions on a limited domain (+-256)
+  static __device__ __forceinline__ void sincos_phase(float x, float* s, float* c) {
+    const float r = __builtin_amdgcn_fractf(x * 0.159154943091895336f);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+  }
   static __device__ __forceinline__ float fabs(float x) { return fabsf(x); }
   static __device__ __forceinline__ float exp(float x) { return __expf(x); }
   static __device__ __forceinline__ float floor(float x) { return floorf(x); }
@@ -66,6 +74,7 @@ template <> struct M<double> {
     ::sincos(p, s, c);
   }
   static __device__ __forceinline__ void sincos(double x, double* s, double* c) { ::sincos(x, s, c); }
+  static __device__ __forceinline__ void sincos_phase(double x, double* s, double* c) { ::sincos(x, s, c); }
   static __device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
   static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
   static __device__ __forceinline__ double floor(double x) { return ::floor(x); }

@@ -439,7 +439,7 @@ __device__ __forceinline__ void reverse_suffix(const STab<R>* stab, const Specie
   xn = cadd(cscale(ym, r10), cscale(yn, r11));
   const R nrm2 = cnorm(xm) + cnorm(xn);
   if (nrm2 > R(0) && M<R>::sqrt(nrm2) > static_cast<R>(sizeof(R) == 8 ? 1e-300 : 1e-30)) {
-    const R inv = R(1) / M<R>::sqrt(nrm2);
+    const R inv = M<R>::rsqrt(nrm2);
     Erm = cscale(xm, inv);
     Ern = cscale(xn, inv);
   } else {
@@ -939,7 +939,7 @@ __device__ __forceinline__ R est_optical_depth(const SceneView<R>& v, int n_laye
       const bool up = dz < R(0);
       const bool has = up ? (layer > 0 && v.L[layer].z_min >= zt) : (v.L[layer].nbr & 1) && v.L[layer].z_max <= zt;
       if (has) {
-        const R sd = ((up ? v.L[layer].z_min : v.L[layer].z_max) - p.z) / dz;
+        const R sd = qdiv((up ? v.L[layer].z_min : v.L[layer].z_max) - p.z, dz);
         if (sd > eps && sd < remaining) seg = sd;
       }
     }
@@ -1057,14 +1057,14 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
                      e1.z * a1 + e2.z * a2 - s_in.z * c_th};
       // intersect_plane (detector.cpp:36-47) with the sensor plane, normal +z
       if (M<R>::fabs(so.z) < static_cast<R>(1e-14)) continue;
-      const R L = (zp - pos.z) / so.z;
+      const R L = qdiv(zp - pos.z, so.z);
       if (!(L > R(0))) continue;
       const R Tr = M<R>::exp(-est_optical_depth(v, p.n_layers, vlayer, pos, so, L));
       if (Tr < static_cast<R>(1e-20)) continue;
       const V3<R> pv = cross_unit(Pc.s, so);
       const R pnrm = M<R>::sqrt(dot(pv, pv));
       if (pnrm < static_cast<R>(1e-12)) continue;
-      const V3<R> ph_hat = vscale(pv, R(1) / pnrm);
+      const V3<R> ph_hat = vscale(pv, qdiv(R(1), pnrm));
       const R sin_phi = -dot(Pc.m, ph_hat), cos_phi = dot(Pc.n, ph_hat);
       const R th = M<R>::acos(clamp1(dot(Pc.s, so)));
       cx<R> s1, s2;
@@ -1074,13 +1074,13 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
       const R cc = cos_phi * cos_phi, ss = sin_phi * sin_phi;
       const R F = Emm * (s22 * cc + s11 * ss) + Enn * (s22 * ss + s11 * cc) + E12 * (s22 - s11) * sin_phi * cos_phi;
       if (F < tiny) continue;
-      const R w_bin = w * Tr * ((F / Sv.pi_inorm) * (dom * dphi));
+      const R w_bin = w * Tr * (qdiv(F, Sv.pi_inorm) * (dom * dphi));
       if (w_bin < tiny) continue;
       int t_idx = 0;  // time_bin (detector.cpp:1182-1190); photon speed 1 in path units
       if (dt > R(0)) {
         const R arr = t0 + (gpl + L);
         if (arr < R(0) || arr >= tmax) continue;
-        t_idx = static_cast<int>(arr / dt) + 1;
+        t_idx = static_cast<int>(qdiv(arr, dt)) + 1;
         if (t_idx >= sd.n2) continue;
       }
       const R amp = M<R>::sqrt(w_bin);
@@ -1098,7 +1098,7 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
         Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
                 stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
       }
-      const R fi = amp / M<R>::sqrt(F);
+      const R fi = qdiv(amp, M<R>::sqrt(F));
       const cx<R> um = cadd(cscale(Em, cos_phi), cscale(En, sin_phi));
       const cx<R> un = cadd(cscale(Em, -sin_phi), cscale(En, cos_phi));
       const cx<R> fm = cscale(cmul(s2, um), fi), fn = cscale(cmul(s1, un), fi);
@@ -1112,7 +1112,7 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
         const R arg = lc.k0 * ((so.x + s_in.x) * (pos.x - r1.x) + (so.y + s_in.y) * (pos.y - r1.y) +
                                (so.z + s_in.z) * (pos.z - r1.z));
         R sa, ca;
-        M<R>::sincos(arg, &sa, &ca);
+        M<R>::sincos_phase(arg, &sa, &ca);
         const cx<R> pp{ca * amp, sa * amp};
         rm = cmul(rm, pp);
         rn = cmul(rn, pp);
@@ -2046,7 +2046,7 @@ __global__ void __launch_bounds__(256) k_crossings(KParams<R> p) {
           const R arg =
               lc.k0 * ((P.s.x + lc.ldir[0]) * d.x + (P.s.y + lc.ldir[1]) * d.y + (P.s.z + lc.ldir[2]) * d.z);
           R sa, ca;
-          M<R>::sincos(arg, &sa, &ca);
+          M<R>::sincos_phase(arg, &sa, &ca);
           const cx<R> pp{ca * ws, sa * ws};
           rm = cmul(rm, pp);
           rn = cmul(rn, pp);
@@ -2260,6 +2260,17 @@ __global__ void k_azimuth_probe(int n, const double* abc, const uint32_t* words,
     out[3 * i] = static_cast<double>(cph);
     out[3 * i + 1] = static_cast<double>(sph);
     out[3 * i + 2] = static_cast<double>(F);
+  }
+}
+
+// sin and cos of FP32 path phases as the CBS reverse-path factor forms them.
+__global__ void k_phase_probe(int n, const float* x, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    float s, c;
+    M<float>::sincos_phase(x[i], &s, &c);
+    out[2 * i] = s;
+    out[2 * i + 1] = c;
   }
 }
 

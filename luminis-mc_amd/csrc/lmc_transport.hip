@@ -1685,6 +1685,22 @@ int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1, u
   return LMC_OK;
 }
 
+int lmc_probe_phase(int32_t n, const float* x, float* out) {
+  if (n <= 0 || n > (1 << 24) || !x || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
+  int rc = ensure_device();
+  if (rc) return rc;
+  float *dx = nullptr, *dout = nullptr;
+  LMC_HIP(hipMalloc(&dx, sizeof(float) * n));
+  LMC_HIP(hipMalloc(&dout, sizeof(float) * 2 * n));
+  LMC_HIP(hipMemcpy(dx, x, sizeof(float) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_phase_probe, dim3((n + 255) / 256), dim3(256), 0, 0, n, dx, dout);
+  LMC_HIP(hipGetLastError());
+  LMC_HIP(hipMemcpy(out, dout, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+  LMC_HIP(hipFree(dx));
+  LMC_HIP(hipFree(dout));
+  return LMC_OK;
+}
+
 int lmc_probe_stream(uint32_t k0, uint32_t k1, uint64_t photon, int32_t n, uint32_t* out) {
   if (n <= 0 || n > (1 << 20) || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
   int rc = ensure_device();

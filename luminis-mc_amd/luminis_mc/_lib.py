@@ -208,6 +208,7 @@ def lib():
                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     L.lmc_probe_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_uint64, c_int32,
                                    ctypes.POINTER(ctypes.c_uint32)]
+    L.lmc_probe_phase.argtypes = [c_int32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     if L.lmc_abi_version() != 2:
         raise ImportError("libluminis_hip.so ABI version mismatch")
     _lib = L
