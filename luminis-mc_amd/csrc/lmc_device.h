@@ -113,6 +113,23 @@ template <class R> __device__ __forceinline__ V3<R> safe_unit(V3<R> v, V3<R> fb)
   return vscale(v, M<R>::rsqrt(n2));
 }
 
+// safe_unit(cross_unit(a, b), fb): the unit cross product, or fb normalised
+// when the cross product is exactly zero.  FP32: one normalisation (the second
+// one of the composed form rescales a unit vector by rsqrt(1 +- ulps));
+// FP64: the composed form, operation for operation.
+template <class R> __device__ __forceinline__ V3<R> unit_cross_or(V3<R> a, V3<R> b, V3<R> fb) {
+  if constexpr (sizeof(R) == 8) {
+    return safe_unit(cross_unit(a, b), fb);
+  } else {
+    const V3<R> r{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    const R n2 = dot(r, r);
+    if (n2 > R(0)) return vscale(r, M<R>::rsqrt(n2));
+    const R f2 = dot(fb, fb);
+    if (f2 < R(1e-20)) return {R(1), R(0), R(0)};
+    return vscale(fb, M<R>::rsqrt(f2));
+  }
+}
+
 // Local frame: rows m, n, s (photon.hpp P_local).
 template <class R> struct Frame {
   V3<R> m, n, s;

@@ -378,9 +378,9 @@ __device__ __forceinline__ void reverse_prefix(const LaunchConst<R>& lc, const S
   const R th_a = M<R>::acos(clamp1(dot(s0, s_out_a)));
   cx<R> a1, a2;
   s_lookup(stab, spn, th_a, a1, a2);
-  const V3<R> np = safe_unit(cross_unit(s0, s_out_a), n0);
-  const V3<R> mp_in = safe_unit(cross_unit(np, s0), m0);
-  const V3<R> mp_out = safe_unit(cross_unit(np, s_out_a), Pq.m);
+  const V3<R> np = unit_cross_or(s0, s_out_a, n0);
+  const V3<R> mp_in = unit_cross_or(np, s0, m0);
+  const V3<R> mp_out = unit_cross_or(np, s_out_a, Pq.m);
   const cx<R> E0m{lc.pol[0], lc.pol[1]}, E0n{lc.pol[2], lc.pol[3]};
   // E = S_a * rot2(mp_in, np, m0, n0) * E0
   R r00 = dot(mp_in, m0), r01 = dot(mp_in, n0), r10 = dot(np, m0), r11 = dot(np, n0);
@@ -423,9 +423,9 @@ __device__ __forceinline__ void reverse_suffix(const STab<R>* stab, const Specie
   cx<R> c1, c2;
   s_lookup(stab, sp1, th_c, c1, c2);
   const V3<R> n1neg{-P1.n.x, -P1.n.y, -P1.n.z};
-  const V3<R> npp = safe_unit(cross_unit(s_in_c, Pn.s), n1neg);
-  const V3<R> mpp_in = safe_unit(cross_unit(npp, s_in_c), P1.m);
-  const V3<R> mpp_out = safe_unit(cross_unit(npp, Pn.s), Pn.m);
+  const V3<R> npp = unit_cross_or(s_in_c, Pn.s, n1neg);
+  const V3<R> mpp_in = unit_cross_or(npp, s_in_c, P1.m);
+  const V3<R> mpp_out = unit_cross_or(npp, Pn.s, Pn.m);
   R r00 = dot(mpp_in, P1.m), r01 = dot(mpp_in, n1neg), r10 = dot(npp, P1.m), r11 = dot(npp, n1neg);
   cx<R> ym = cadd(cscale(xm, r00), cscale(xn, r01));
   cx<R> yn = cadd(cscale(xm, r10), cscale(xn, r11));
@@ -1025,9 +1025,11 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
   const R zp = static_cast<R>(v.planes[sd.plane].z);
   const int64_t tplane = static_cast<int64_t>(Nth) * Nph, plane = tplane * sd.n2;
   const bool in_lds = sd.est_lds >= 0;
-  double* lg = in_lds ? v.lgrid + sd.est_lds : nullptr;
+  lds_double* lgs = in_lds ? (lds_double*)(v.lgrid + sd.est_lds) : nullptr;
   double* hg = p.out + sd.out_off;
   const int nb = sd.est_imax * Nph;
+  const int it_l = lane / Nph, jp_l = lane - it_l * Nph;  // this lane's first bin
+  const int d_it = 64 / Nph, d_jp = 64 - d_it * Nph;
   while (todo) {
     const int l = __ffsll(static_cast<unsigned long long>(todo)) - 1;
     todo &= todo - 1;
@@ -1048,8 +1050,16 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
     const SpeciesDev<R>& Sv = v.S[vsp];
     const SpeciesDev<R>& S1 = v.S[vsp1];
     const R Emm = cnorm(Em), Enn = cnorm(En), E12 = R(2) * (Em.r * En.r + Em.i * En.i);
+    int it_n = it_l, jp_n = jp_l;
     for (int b = lane; b < nb; b += 64) {
-      const int it = b / Nph, jp = b - it * Nph;
+      // bin b = it * Nph + jp; this lane's next bin is 64 further on
+      const int it = it_n, jp = jp_n;
+      it_n += d_it;
+      jp_n += d_jp;
+      if (jp_n >= Nph) {
+        jp_n -= Nph;
+        ++it_n;
+      }
       const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
       const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
       const R a1 = s_th * c_ph, a2 = s_th * s_ph;
@@ -1120,11 +1130,22 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
         Ery = cadd(cscale(rm, Pe.m.y), cscale(rn, Pe.n.y));
       }
       const cx<R> Ex = cadd(Efx, Erx), Ey = cadd(Efy, Ery);
-      const int64_t off = t_idx * tplane + b;
+      R val[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const R val = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
-        acc_add(lg, hg, in_lds, q * plane + off, static_cast<double>(val));
+      for (int q = 0; q < 8; ++q)
+        val[q] = q < 4 ? stokes(q, Ex, Ey) : stokes(q - 4, Efx, Efy) + stokes(q - 4, Erx, Ery);
+      if (in_lds) {  // wave-uniform; LDS offsets in 32 bits
+        lds_double* a = lgs + (static_cast<int>(t_idx) * static_cast<int>(tplane) + b);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (val[q] != R(0))
+            __hip_atomic_fetch_add(a + q * static_cast<int>(plane), static_cast<double>(val[q]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else {
+        double* a = hg + (t_idx * tplane + b);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (val[q] != R(0)) unsafeAtomicAdd(a + q * plane, static_cast<double>(val[q]));
       }
     }
   }
