@@ -1105,8 +1105,12 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
                 ctc * Pc.m.z + cts * Pc.n.z - st * Pc.s.z};
         Pe.n = {-sin_phi * Pc.m.x + cos_phi * Pc.n.x, -sin_phi * Pc.m.y + cos_phi * Pc.n.y,
                 -sin_phi * Pc.m.z + cos_phi * Pc.n.z};
-        Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
-                stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
+        // the rotated direction is the forced one, so (FP32) so itself
+        if constexpr (sizeof(R) == 4)
+          Pe.s = so;
+        else
+          Pe.s = {stc * Pc.m.x + sts * Pc.n.x + ct * Pc.s.x, stc * Pc.m.y + sts * Pc.n.y + ct * Pc.s.y,
+                  stc * Pc.m.z + sts * Pc.n.z + ct * Pc.s.z};
       }
       const R fi = qdiv(amp, M<R>::sqrt(F));
       const cx<R> um = cadd(cscale(Em, cos_phi), cscale(En, sin_phi));
@@ -1137,10 +1141,9 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
       if (in_lds) {  // wave-uniform; LDS offsets in 32 bits
         lds_double* a = lgs + (static_cast<int>(t_idx) * static_cast<int>(tplane) + b);
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (val[q] != R(0))
-            __hip_atomic_fetch_add(a + q * static_cast<int>(plane), static_cast<double>(val[q]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int q = 0; q < 8; ++q)  // no zero test: LDS adds are cheap and the values are almost never zero
+          __hip_atomic_fetch_add(a + q * static_cast<int>(plane), static_cast<double>(val[q]), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
       } else {
         double* a = hg + (t_idx * tplane + b);
 #pragma unroll
