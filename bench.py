@@ -51,8 +51,8 @@ PEAK_FP32_NONPACKED = 78.6     # single-issue VALU FP32 (this build emits no pac
 # resident; the writes are the light sensors' FP64 deposits (planar fluence,
 # far-field fluence: 6.4 per photon, one 32-byte sector each) and the few CBS
 # crossing records handed to the sensor kernel.
-HBM_FETCH_B_PER_PHOTON = 1652.7 * 1024 * 2.00 / 4e6
-HBM_WRITE_B_PER_PHOTON = 8.0713e5 * 1024 * 1.00 / 4e6
+HBM_FETCH_B_PER_PHOTON = 1663.1 * 1024 * 2.00 / 4e6
+HBM_WRITE_B_PER_PHOTON = 8.0712e5 * 1024 * 1.00 / 4e6
 
 
 def log(msg: str) -> None:
