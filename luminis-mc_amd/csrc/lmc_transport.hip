@@ -819,7 +819,7 @@ int run_impl(lmc_scene* s, DevTables<R>& T, const lmc_run_desc* r, const uint64_
   // without plane code in the flight loop only where a flight crosses a plane
   // only when it leaves the sample
   const int aux_mode =
-      abs_run ? 2 : ((has_stats || rec_run || est || s->flight_planes || sizeof(R) != 4 || kXo || kAdj) ? 1 : 0);
+      abs_run ? 2 : ((has_stats || rec_run || est || s->flight_planes || sizeof(R) != 4 || kXo) ? 1 : 0);
   // the light sensors' crossings: deposited by the transport kernel or queued
   p.direct = deposits_direct(sizeof(R) == 8, kXo, kAdj, aux_mode, batch) ? 1 : 0;
   auto pick = [&](bool wide, bool lds_t) -> TransportFn<R> {

@@ -17,11 +17,11 @@ namespace {
 // The transport kernel of a launch: the estimator, 1024-thread or
 // 512/256-thread body; a batch or a single scene; with or without reverse-path
 // tracking; tables in LDS or read through the caches.  kAux as in the
-// kernels: 2 (Absorption grid), 0 for FP32 Philox forward runs only;
+// kernels: 2 (Absorption grid), 0 for FP32 Philox runs only;
 // nullptr where a combination does not exist.
 template <class R, bool kXo, bool kAdj, int kAux>
 TransportFn<R> pick_transport(bool est, bool wide, bool lds_tables, bool batch, bool track) {
-  if constexpr (kAux == 0 && (sizeof(R) != 4 || kXo || kAdj)) {
+  if constexpr (kAux == 0 && (sizeof(R) != 4 || kXo)) {
     return nullptr;
   } else {
     if (est) {
