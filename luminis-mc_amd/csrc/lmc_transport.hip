@@ -389,6 +389,10 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
       c.ln[j] = static_cast<R>(Ls.n[j]);
     }
     for (int j = 0; j < 4; ++j) c.pol[j] = static_cast<R>(Ls.pol[j]);
+    // handedness of the launch frame, (m x n) . s: every photon frame keeps it
+    const double hx = Ls.m[1] * Ls.n[2] - Ls.m[2] * Ls.n[1], hy = Ls.m[2] * Ls.n[0] - Ls.m[0] * Ls.n[2],
+                 hz = Ls.m[0] * Ls.n[1] - Ls.m[1] * Ls.n[0];
+    c.hand = (hx * Ls.dir[0] + hy * Ls.dir[1] + hz * Ls.dir[2]) < 0.0 ? R(-1) : R(1);
     c.k0 = static_cast<R>(2.0 * M_PI / Ls.wavelength);
     c.k0d = 2.0 * M_PI / Ls.wavelength;
     c.sigma = Ls.sigma;
