@@ -1046,7 +1046,9 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
   const V3<R> e1{lc.lm[0], lc.lm[1], lc.lm[2]}, e2{lc.ln[0], lc.ln[1], lc.ln[2]};
   const V3<R> s_in{lc.ldir[0], lc.ldir[1], lc.ldir[2]};
   const int Nth = sd.n0, Nph = sd.n1;
-  const R* gc = p.aux + sd.aux;  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each)
+  // cos th, sin th, dOmega (Nth each), cos ph, sin ph (Nph each); then, from
+  // the next multiple of 4, per bin (so.x, so.y, so.z, dOmega dphi) in FP32
+  const R* gc = p.aux + sd.aux;
   const R dphi = static_cast<R>(sd.g[4]);
   const R tmax = static_cast<R>(sd.g[2]), dt = static_cast<R>(sd.g[5]);
   const R zp = static_cast<R>(v.planes[sd.plane].z);
@@ -1055,6 +1057,7 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
   lds_double* lgs = in_lds ? (lds_double*)(v.lgrid + sd.est_lds) : nullptr;
   double* hg = p.out + sd.out_off;
   const int nb = sd.est_imax * Nph;
+  const int so_off = (sd.aux + 3 * Nth + 2 * Nph + 3) & ~3;
   const int it_l = lane / Nph, jp_l = lane - it_l * Nph;  // this lane's first bin
   const int d_it = 64 / Nph, d_jp = 64 - d_it * Nph;
   while (todo) {
@@ -1087,11 +1090,20 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
         jp_n -= Nph;
         ++it_n;
       }
-      const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
-      const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
-      const R a1 = s_th * c_ph, a2 = s_th * s_ph;
-      const V3<R> so{e1.x * a1 + e2.x * a2 - s_in.x * c_th, e1.y * a1 + e2.y * a2 - s_in.y * c_th,
-                     e1.z * a1 + e2.z * a2 - s_in.z * c_th};
+      V3<R> so;
+      R dwt;  // dOmega dphi
+      if constexpr (sizeof(R) == 4) {  // the bin's precomputed record
+        const float4 g4 = reinterpret_cast<const float4*>(p.aux + so_off)[b];
+        so = {g4.x, g4.y, g4.z};
+        dwt = g4.w;
+      } else {
+        const R c_th = gc[it], s_th = gc[Nth + it], dom = gc[2 * Nth + it];
+        const R c_ph = gc[3 * Nth + jp], s_ph = gc[3 * Nth + Nph + jp];
+        const R a1 = s_th * c_ph, a2 = s_th * s_ph;
+        so = {e1.x * a1 + e2.x * a2 - s_in.x * c_th, e1.y * a1 + e2.y * a2 - s_in.y * c_th,
+              e1.z * a1 + e2.z * a2 - s_in.z * c_th};
+        dwt = dom * dphi;
+      }
       // intersect_plane (detector.cpp:36-47) with the sensor plane, normal +z
       if (M<R>::fabs(so.z) < static_cast<R>(1e-14)) continue;
       const R L = qdiv(zp - pos.z, so.z);
@@ -1111,7 +1123,7 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
       const R cc = cos_phi * cos_phi, ss = sin_phi * sin_phi;
       const R F = Emm * (s22 * cc + s11 * ss) + Enn * (s22 * ss + s11 * cc) + E12 * (s22 - s11) * sin_phi * cos_phi;
       if (F < tiny) continue;
-      const R w_bin = w * Tr * (qdiv(F, Sv.pi_inorm) * (dom * dphi));
+      const R w_bin = w * Tr * (qdiv(F, Sv.pi_inorm) * dwt);
       if (w_bin < tiny) continue;
       int t_idx = 0;  // time_bin (detector.cpp:1182-1190); photon speed 1 in path units
       if (dt > R(0)) {

@@ -1318,6 +1318,23 @@ int add_scene(lmc_scene* s, const lmc_scene_desc* d, bool privatise) {
             const double ph = sd.n_phi_explicit > 0 ? d->tables[sd.phi_offset + jp] : (jp + 0.5) * dph;
             eg.push_back(std::sin(ph));
           }
+          // FP32 kernels read per bin the forced direction in the lab frame
+          // (e1 a1 + e2 a2 - s_in cos th, the laser's frame) and dOmega dphi:
+          // 16-byte records from the next multiple of 4
+          while (eg.size() % 4) eg.push_back(0.0);
+          const double* e1 = d->laser.m;
+          const double* e2 = d->laser.n;
+          const double* s_in = d->laser.dir;
+          for (int it = 0; it < sd.n[0]; ++it) {
+            const double cth = std::cos((it + 0.5) * dth), sth = std::sin((it + 0.5) * dth);
+            const double dom = std::cos(it * dth) - std::cos((it + 1) * dth);
+            for (int jp = 0; jp < sd.n[1]; ++jp) {
+              const double ph = sd.n_phi_explicit > 0 ? d->tables[sd.phi_offset + jp] : (jp + 0.5) * dph;
+              const double a1 = sth * std::cos(ph), a2 = sth * std::sin(ph);
+              for (int j = 0; j < 3; ++j) eg.push_back(e1[j] * a1 + e2[j] * a2 - s_in[j] * cth);
+              eg.push_back(dom * dph);
+            }
+          }
           // the transport kernel's LDS copy of the grid when it is small (<= 48 KiB in all)
           if ((s->est_lds_n + sd.out_len) * 8 <= 49152) {
             o.est_lds = s->est_lds_n;
