@@ -1110,12 +1110,29 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
       if (!(L > R(0))) continue;
       const R Tr = M<R>::exp(-est_optical_depth(v, p.n_layers, vlayer, pos, so, L));
       if (Tr < static_cast<R>(1e-20)) continue;
-      const V3<R> pv = cross_unit(Pc.s, so);
-      const R pnrm = M<R>::sqrt(dot(pv, pv));
-      if (pnrm < static_cast<R>(1e-12)) continue;
-      const V3<R> ph_hat = vscale(pv, qdiv(R(1), pnrm));
-      const R sin_phi = -dot(Pc.m, ph_hat), cos_phi = dot(Pc.n, ph_hat);
-      const R th = M<R>::acos(clamp1(dot(Pc.s, so)));
+      R sin_phi, cos_phi, th, ct_v = R(0), st_v = R(0);
+      if constexpr (sizeof(R) == 4) {
+        // FP32: both angles from so's components in the vertex frame, x = m.so,
+        // y = n.so: |s x so| = |(x, y)|, and with m x n = h s the reference's
+        // -m.unit(s x so) and n.unit(s x so) are h y / |(x, y)|, h x / |(x, y)|
+        const R x = dot(Pc.m, so), y = dot(Pc.n, so);
+        const R p2 = x * x + y * y;
+        if (!(p2 > R(0))) continue;
+        const R ip = M<R>::rsqrt(p2);
+        cos_phi = lc.hand * x * ip;
+        sin_phi = lc.hand * y * ip;
+        ct_v = clamp1(dot(Pc.s, so));
+        st_v = p2 * ip;
+        th = M<R>::acos(ct_v);
+      } else {
+        const V3<R> pv = cross_unit(Pc.s, so);
+        const R pnrm = M<R>::sqrt(dot(pv, pv));
+        if (pnrm < static_cast<R>(1e-12)) continue;
+        const V3<R> ph_hat = vscale(pv, qdiv(R(1), pnrm));
+        sin_phi = -dot(Pc.m, ph_hat);
+        cos_phi = dot(Pc.n, ph_hat);
+        th = M<R>::acos(clamp1(dot(Pc.s, so)));
+      }
       cx<R> s1, s2;
       s_lookup(v.stab, Sv, th, s1, s2);
       // phase_F (detector.cpp:1012-1023)
@@ -1135,8 +1152,8 @@ __device__ void estimate_wave(const KParams<R>& p, const SceneView<R>& v, const 
       const R amp = M<R>::sqrt(w_bin);
       // exit frame (scatter_frame, detector.cpp:1027-1044) and the normalised
       // forward field (apply_scatter_normalized, :121-161)
-      R st, ct;
-      M<R>::sincos(th, &st, &ct);
+      R st = st_v, ct = ct_v;
+      if constexpr (sizeof(R) == 8) M<R>::sincos(th, &st, &ct);
       Frame<R> Pe;
       {
         const R ctc = ct * cos_phi, cts = ct * sin_phi, stc = st * cos_phi, sts = st * sin_phi;
