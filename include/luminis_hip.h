@@ -386,6 +386,14 @@ int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1,
  * 0xFFFFFFFE, 0). */
 int lmc_probe_stream(uint32_t k0, uint32_t k1, uint64_t photon, int32_t n, uint32_t* out);
 
+/* Stage C of the CBS reverse field on device (reverse_field_suffix,
+ * detector.cpp:1123-1161) for given frames: frames[20i..] = m1, s1 (the first
+ * scatter's frame, n1 = s1 x m1), m_n, n_n, s_n (the exit frame), the prefix
+ * field (xm.re, xm.im, xn.re, xn.im) and the launch frame's handedness
+ * (+1: m x n = s, -1: m x n = -s); the amplitudes are the constants S1 = 0.6 +
+ * 0.2i, S2 = 0.3 - 0.5i; out[4i..] = Erm.re, Erm.im, Ern.re, Ern.im. */
+int lmc_probe_reverse(int32_t n, int32_t precision, const double* frames, double* out);
+
 /* FP32 path phase on device: out[2i], out[2i+1] = sin x[i], cos x[i] as the
  * FP32 CBS reverse-path factor exp(i k0 (s_out + s_in).(r_n - r_1)) forms
  * them (detector.cpp:1231-1310), for phases of any size. */

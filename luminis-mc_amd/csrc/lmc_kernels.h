@@ -2350,6 +2350,44 @@ __global__ void k_azimuth_probe(int n, const double* abc, const uint32_t* words,
   }
 }
 
+// Stage C of the reverse field on given frames: in[20 i ..] = m1, s1 (P1 =
+// frame_of(m1, s1)), m_n, n_n, s_n, (xm, xn) as 4 reals, the launch frame's
+// handedness; constant amplitudes S1 = 0.6 + 0.2i, S2 = 0.3 - 0.5i on a
+// two-node table; out[4 i ..] = Erm, Ern.
+template <class R>
+__global__ void k_reverse_probe(int n, const double* in, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double* q = in + 20 * static_cast<int64_t>(i);
+    auto v3 = [&](int o) { return V3<R>{static_cast<R>(q[o]), static_cast<R>(q[o + 1]), static_cast<R>(q[o + 2])}; };
+    const Frame<R> P1 = frame_of(v3(0), v3(3));
+    Frame<R> Pn;
+    Pn.m = v3(6);
+    Pn.n = v3(9);
+    Pn.s = v3(12);
+    const cx<R> xm{static_cast<R>(q[15]), static_cast<R>(q[16])}, xn{static_cast<R>(q[17]), static_cast<R>(q[18])};
+    STab<R> tb[2];
+    for (int k = 0; k < 2; ++k) {
+      if constexpr (sizeof(R) == 4) {
+        tb[k].v = make_float4(0.6f, 0.2f, 0.3f, -0.5f);
+      } else {
+        tb[k].a = make_double2(0.6, 0.2);
+        tb[k].b = make_double2(0.3, -0.5);
+      }
+    }
+    SpeciesDev<R> sp{};
+    sp.s_n = 2;
+    sp.s_off = 0;
+    sp.inv_ds = static_cast<R>(1.0 / 3.14159265358979323846);
+    cx<R> rm, rn;
+    reverse_suffix(tb, sp, P1, Pn, static_cast<R>(q[19]), xm, xn, rm, rn);
+    out[4 * i] = rm.r;
+    out[4 * i + 1] = rm.i;
+    out[4 * i + 2] = rn.r;
+    out[4 * i + 3] = rn.i;
+  }
+}
+
 // sin and cos of FP32 path phases as the CBS reverse-path factor forms them.
 __global__ void k_phase_probe(int n, const float* x, float* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -1706,6 +1706,26 @@ int lmc_probe_philox(int32_t n, const uint32_t* ctr, uint32_t k0, uint32_t k1, u
   return LMC_OK;
 }
 
+int lmc_probe_reverse(int32_t n, int32_t precision, const double* frames, double* out) {
+  if (n <= 0 || n > (1 << 22) || !frames || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
+  if (precision != LMC_PREC_FP32 && precision != LMC_PREC_FP64) return fail(LMC_ERR_INVALID, "unknown precision");
+  int rc = ensure_device();
+  if (rc) return rc;
+  double *din = nullptr, *dout = nullptr;
+  LMC_HIP(hipMalloc(&din, sizeof(double) * 20 * n));
+  LMC_HIP(hipMalloc(&dout, sizeof(double) * 4 * n));
+  LMC_HIP(hipMemcpy(din, frames, sizeof(double) * 20 * n, hipMemcpyHostToDevice));
+  if (precision == LMC_PREC_FP32)
+    hipLaunchKernelGGL(k_reverse_probe<float>, dim3((n + 255) / 256), dim3(256), 0, 0, n, din, dout);
+  else
+    hipLaunchKernelGGL(k_reverse_probe<double>, dim3((n + 255) / 256), dim3(256), 0, 0, n, din, dout);
+  LMC_HIP(hipGetLastError());
+  LMC_HIP(hipMemcpy(out, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  LMC_HIP(hipFree(din));
+  LMC_HIP(hipFree(dout));
+  return LMC_OK;
+}
+
 int lmc_probe_phase(int32_t n, const float* x, float* out) {
   if (n <= 0 || n > (1 << 24) || !x || !out) return fail(LMC_ERR_INVALID, "bad probe arguments");
   int rc = ensure_device();

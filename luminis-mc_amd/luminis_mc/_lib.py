@@ -209,6 +209,7 @@ def lib():
     L.lmc_probe_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, c_uint64, c_int32,
                                    ctypes.POINTER(ctypes.c_uint32)]
     L.lmc_probe_phase.argtypes = [c_int32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.lmc_probe_reverse.argtypes = [c_int32, c_int32, c_dp, c_dp]
     if L.lmc_abi_version() != 2:
         raise ImportError("libluminis_hip.so ABI version mismatch")
     _lib = L
