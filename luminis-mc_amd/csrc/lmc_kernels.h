@@ -1514,10 +1514,12 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       const R ch = qdiv(R(1) + gg - frac * frac, R(2) * g);
       c = hg ? ch : c;
       if constexpr (kAdj) {
+        // d ln p_HG / dg at the sampled cosine (hardware reciprocals in FP32)
         const R cc = clamp1(ch);
+        const R sg = -R(2) * qdiv(g, R(1) - gg) - R(3) * qdiv(g - cc, R(1) + gg - R(2) * g * cc);
 #pragma unroll
         for (int l = 0; l < kMaxGradLayers; ++l)
-          if (hg && l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - cc) / (R(1) + gg - R(2) * g * cc);
+          if (hg && l == ph.layer) ph.aG[l] += sg;
       }
     }
     R t = R(0);
