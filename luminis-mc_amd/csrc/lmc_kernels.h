@@ -194,6 +194,10 @@ template <class R, bool kXo> struct Photon {
   uint32_t w_th, w_ph, w_az;
   XoState<kXo> xo;
   R aL[kMaxGradLayers], aN[kMaxGradLayers], aG[kMaxGradLayers];
+  // the current layer's path length, scatter count and HG score not yet added
+  // into aL, aN, aG[layer] (added when the photon changes layer): one add per
+  // event instead of a masked add per layer slot
+  R cL, cN, cG;
 };
 
 template <class R> __device__ __forceinline__ V3<R> cross3(V3<R> a, V3<R> b) {
@@ -507,10 +511,12 @@ __device__ __forceinline__ void adj_scores(const KParams<R>& p, const SceneView<
 #pragma unroll
   for (int l = 0; l < kMaxGradLayers; ++l) {
     const R mus = l < p.n_layers ? v.L[l].mu_s : R(1);
-    const R Ll = ph.aL[l] - (l == ph.layer ? corr : R(0));
-    score[3 * l + 0] = (mus > R(0) ? ph.aN[l] / mus : R(0)) - Ll;
+    const bool cur = l == ph.layer;
+    const R Ll = ph.aL[l] + (cur ? ph.cL - corr : R(0));
+    const R Nl = ph.aN[l] + (cur ? ph.cN : R(0));
+    score[3 * l + 0] = (mus > R(0) ? Nl / mus : R(0)) - Ll;
     score[3 * l + 1] = -Ll;
-    score[3 * l + 2] = ph.aG[l];
+    score[3 * l + 2] = ph.aG[l] + (cur ? ph.cG : R(0));
   }
 }
 
@@ -1321,13 +1327,29 @@ __device__ __forceinline__ void launch(const KParams<R>& p, const LaunchConst<R>
     ph.aN[l] = R(0);
     ph.aG[l] = R(0);
   }
+  ph.cL = R(0);
+  ph.cN = R(0);
+  ph.cG = R(0);
 }
 
 template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void add_path(Photon<R, kXo>& ph, R d) {
+  if constexpr (kAdj) ph.cL += d;
+}
+
+// the current layer's score shares into its slots, before the photon leaves it
+template <class R, bool kXo, bool kAdj> __device__ __forceinline__ void flush_scores(Photon<R, kXo>& ph) {
   if constexpr (kAdj) {
 #pragma unroll
-    for (int l = 0; l < kMaxGradLayers; ++l)
-      if (l == ph.layer) ph.aL[l] += d;
+    for (int l = 0; l < kMaxGradLayers; ++l) {
+      if (l == ph.layer) {
+        ph.aL[l] += ph.cL;
+        ph.aN[l] += ph.cN;
+        ph.aG[l] += ph.cG;
+      }
+    }
+    ph.cL = R(0);
+    ph.cN = R(0);
+    ph.cG = R(0);
   }
 }
 
@@ -1417,6 +1439,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       if (!has) break;
       const R rem = (step - seg) * Lc.mu_t;
       on_iface = sizeof(R) == 4;
+      flush_scores<R, kXo, kAdj>(ph);
       ph.layer += up ? 1 : -1;
       const R mt = v.L[ph.layer].mu_t;
       step = mt > R(0) ? qdiv(rem, mt) : rem;
@@ -1517,9 +1540,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
         // d ln p_HG / dg at the sampled cosine (hardware reciprocals in FP32)
         const R cc = clamp1(ch);
         const R sg = -R(2) * qdiv(g, R(1) - gg) - R(3) * qdiv(g - cc, R(1) + gg - R(2) * g * cc);
-#pragma unroll
-        for (int l = 0; l < kMaxGradLayers; ++l)
-          if (hg && l == ph.layer) ph.aG[l] += sg;
+        if (hg) ph.cG += sg;
       }
     }
     R t = R(0);
@@ -1538,11 +1559,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
       const R frac = qdiv(R(1) - gg, R(1) - g + R(2) * g * uth);
       R c = qdiv(R(1) + gg - frac * frac, R(2) * g);
       theta = M<R>::acos(c);
-      if constexpr (kAdj) {
-#pragma unroll
-        for (int l = 0; l < kMaxGradLayers; ++l)
-          if (l == ph.layer) ph.aG[l] += -R(2) * g / (R(1) - gg) - R(3) * (g - c) / (R(1) + gg - R(2) * g * c);
-      }
+      if constexpr (kAdj) ph.cG += -R(2) * g / (R(1) - gg) - R(3) * (g - c) / (R(1) + gg - R(2) * g * c);
     } else if (kind == LMC_PHASE_TABLE_THETA) {
       theta = table_sample(v, S, uth);
     } else if (kind == LMC_PHASE_TABLE_COS) {
@@ -1635,11 +1652,7 @@ __device__ __forceinline__ int transport_event(const KParams<R>& p, const SceneV
     ph.bc = cph;
     ph.bs = sph;
   }
-  if constexpr (kAdj) {
-#pragma unroll
-    for (int l = 0; l < kMaxGradLayers; ++l)
-      if (l == ph.layer) ph.aN[l] += R(1);
-  }
+  if constexpr (kAdj) ph.cN += R(1);
   // weight, roulette, cap (simulation.cpp:578-613)
   ph.events++;
   const R w_in = ph.w;
