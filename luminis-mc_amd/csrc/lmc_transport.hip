@@ -198,7 +198,7 @@ namespace {
 
 constexpr int kGuideN = 4096;
 constexpr int kGuideFineN = 65536;
-constexpr int64_t kGuideBudget = int64_t(1) << 20;  // bytes of guide tables a single scene keeps
+constexpr int64_t kGuideBudget = int64_t(1) << 21;  // bytes of guide tables a single scene keeps
 constexpr int kMaxBatchScenes = 4096;
 
 // compute_I_norm (detector.cpp:105-118): integral of (|S1|^2 + |S2|^2) sin(theta)
@@ -243,8 +243,9 @@ template <class R> int build_tables(lmc_scene* s, DevTables<R>& T) {
   // and a batch keeps only a few scenes' guides in flight.  A single scene's
   // guides all stay in use, so they get the largest power of two entries
   // (4096 to 65536) whose guides fit kGuideBudget bytes in all: 65536 for up
-  // to 8 distinct CDFs, 16384 for a 32-species mixture (whose 65536-entry
-  // guides, 4 MB, no longer stayed in L2: 6 % slower than 4096).
+  // to 16 distinct CDFs, 32768 for a 32-species mixture (2 MB: 2.5 % faster
+  // than 16384 there; its 65536-entry guides, 4 MB, no longer stayed in L2
+  // beside the CDFs and S tables: 6 % slower than 4096).
   int gN = kGuideFineN;
   if (s->scenes.size() == 1) {
     std::set<int64_t> offs;
